@@ -1,0 +1,203 @@
+"""Python view of the gfx950 Goldbach verifier's C ABI (include/goldbach_gpu.h).
+
+The product is the HIP library ``lib/libgoldbach_gpu.so`` built from ``csrc/``;
+this module only binds it with ctypes so tests, the benchmark and Python
+callers can drive it.  Nothing here computes primes: when the library is
+missing, loading fails loudly instead of falling back to any CPU path.
+
+Entry points mirror the reference's per-segment loop (src/goldbach.cu:420-485),
+its sieve (src/goldbach.cu:198-259) and single_check (src/single_check.cu).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libgoldbach_gpu.so")
+HOST_LIB_PATH = os.path.join(PKG_DIR, "lib", "libgoldbach.so")
+BIN_DIR = os.path.join(PKG_DIR, "bin")
+
+GB_OK, GB_EINVAL, GB_ENOMEM, GB_EDEVICE, GB_ENODEV, GB_ERANGE, GB_EAGAIN = range(7)
+GB_WANT_PMIN = 0x1
+GB_FORM_PLAIN = 0
+GB_FORM_BATCHED = 1
+
+# Every symbol include/goldbach_gpu.h declares.
+EXPORTS = (
+    "gb_open", "gb_close", "gb_last_error", "gb_device_count", "gb_device_info",
+    "gb_sieve_window", "gb_verify_segment", "gb_verify_range", "gb_verify_windows",
+    "gb_min_partition_batch",
+)
+
+u64 = ctypes.c_uint64
+u32 = ctypes.c_uint32
+
+
+class GbConfig(ctypes.Structure):
+    _fields_ = [("max_n", u64), ("max_seg_evens", u64), ("phase2_limit", u64), ("mem_budget", u64),
+                ("serve_slots", u32), ("reserved", u32)]
+
+
+class GbSegStats(ctypes.Structure):
+    _fields_ = [("evens", u64), ("phase2", u64), ("failures", u64), ("first_fail_n", u64),
+                ("sum_pmin", u64), ("max_pmin", u64), ("argmax_n", u64), ("kernel_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GbTuning(ctypes.Structure):
+    _fields_ = [("form", u32), ("tile_log2", u32), ("seg_evens", u64), ("wg_per_cu", u32), ("flags", u32)]
+
+
+class GbWindow(ctypes.Structure):
+    _fields_ = [("start", u64), ("evens", u64)]
+
+
+class GoldbachError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+        self.message = message
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load lib/libgoldbach_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    sig = {
+        "gb_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(GbConfig), ctypes.POINTER(vp)]),
+        "gb_close": (ctypes.c_int, [vp]),
+        "gb_last_error": (ctypes.c_char_p, [vp]),
+        "gb_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+        "gb_device_info": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(u64),
+                                          ctypes.POINTER(ctypes.c_int)]),
+        "gb_sieve_window": (ctypes.c_int, [vp, u64, u64, vp]),
+        "gb_verify_segment": (ctypes.c_int, [vp, u64, u64, u64, u32, vp, vp, ctypes.POINTER(GbSegStats)]),
+        "gb_verify_range": (ctypes.c_int, [vp, u64, u64, u64, ctypes.POINTER(GbTuning), ctypes.POINTER(GbSegStats),
+                                           vp]),
+        "gb_verify_windows": (ctypes.c_int, [vp, ctypes.POINTER(GbWindow), u32, u64, u32,
+                                             ctypes.POINTER(GbSegStats), ctypes.POINTER(GbSegStats)]),
+        "gb_min_partition_batch": (ctypes.c_int, [vp, vp, u32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    library().gb_device_count(ctypes.byref(n))
+    return n.value
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+@dataclass
+class SegmentResult:
+    stats: dict
+    verified: Optional[np.ndarray] = None   # uint64 words, bit j of word w <-> n = start + 2(64w+j)
+    pmin: Optional[np.ndarray] = None       # uint32 per even n
+
+
+class Context:
+    """One device context: resident prime tables, stream and segment buffers."""
+
+    def __init__(self, device: int = 0, max_seg_evens: int = 0, phase2_limit: int = 0, max_n: int = 0,
+                 mem_budget: int = 0):
+        self._lib = library()
+        cfg = GbConfig(max_n, max_seg_evens, phase2_limit, mem_budget, 0, 0)
+        h = ctypes.c_void_p()
+        rc = self._lib.gb_open(device, ctypes.byref(cfg), ctypes.byref(h))
+        if rc != GB_OK:
+            raise GoldbachError(rc, self._lib.gb_last_error(None).decode())
+        self._h = h
+
+    # -- lifecycle -------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.gb_close(self._h)
+            self._h = None
+
+    def __enter__(self) -> "Context":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != GB_OK:
+            raise GoldbachError(rc, self._lib.gb_last_error(self._h).decode())
+
+    # -- sieve -----------------------------------------------------------
+    def sieve_window(self, q_low: int, q_high: int) -> np.ndarray:
+        nbits = (q_high - q_low) // 2 + 1 if q_high >= q_low else 0
+        out = np.zeros(max((nbits + 63) // 64, 1), dtype=np.uint64)
+        self._check(self._lib.gb_sieve_window(self._h, q_low, q_high, _ptr(out)))
+        return out
+
+    # -- plain per-segment form -------------------------------------------
+    def verify_segment(self, seg_start: int, count: int, p_small: int, want_pmin: bool = False,
+                       want_verified: bool = True, want_pmin_array: bool = False) -> SegmentResult:
+        st = GbSegStats()
+        ver = np.zeros((count + 63) // 64, dtype=np.uint64) if want_verified else None
+        pm = np.zeros(count, dtype=np.uint32) if want_pmin_array else None
+        flags = GB_WANT_PMIN if (want_pmin or want_pmin_array) else 0
+        self._check(self._lib.gb_verify_segment(self._h, seg_start, count, p_small, flags,
+                                                _ptr(ver) if ver is not None else None,
+                                                _ptr(pm) if pm is not None else None, ctypes.byref(st)))
+        return SegmentResult(st.as_dict(), ver, pm)
+
+    # -- whole range -------------------------------------------------------
+    def verify_range(self, start: int, limit: int, p_small: int, form: int = GB_FORM_BATCHED,
+                     tile_log2: int = 0, seg_evens: int = 0, wg_per_cu: int = 0, want_pmin: bool = False) -> dict:
+        tun = GbTuning(form, tile_log2, seg_evens, wg_per_cu, GB_WANT_PMIN if want_pmin else 0)
+        st = GbSegStats()
+        self._check(self._lib.gb_verify_range(self._h, start, limit, p_small, ctypes.byref(tun), ctypes.byref(st),
+                                              None))
+        return st.as_dict()
+
+    def verify_windows(self, windows: Sequence[tuple], p_small: int, want_pmin: bool = False):
+        arr = (GbWindow * len(windows))(*[GbWindow(int(a), int(e)) for a, e in windows])
+        per = (GbSegStats * len(windows))()
+        tot = GbSegStats()
+        self._check(self._lib.gb_verify_windows(self._h, arr, len(windows), p_small,
+                                                GB_WANT_PMIN if want_pmin else 0, per, ctypes.byref(tot)))
+        return [p.as_dict() for p in per], tot.as_dict()
+
+    # -- single numbers ----------------------------------------------------
+    def min_partition(self, ns) -> np.ndarray:
+        n = np.ascontiguousarray(np.asarray(ns, dtype=np.uint64))
+        out = np.zeros(n.shape[0], dtype=np.uint64)
+        self._check(self._lib.gb_min_partition_batch(self._h, _ptr(n), n.shape[0], _ptr(out)))
+        return out
+
+
+def unpack_bits(words: np.ndarray, nbits: int) -> np.ndarray:
+    """uint64 words (LSB-first) -> uint8 array of nbits flags."""
+    b = np.unpackbits(words.view(np.uint8), bitorder="little")
+    return b[:nbits]

@@ -1,0 +1,732 @@
+// gb_api.cpp -- the C ABI of include/goldbach_gpu.h on top of gb_kernels.hip.
+//
+// Host orchestration only: every prime test, sieve, mark and reduction runs
+// in the HIP kernels.  The host computes launch geometry, builds the static
+// medium-prime work schedule from the device-built prime table, moves the
+// caller's buffers and maps HIP errors to gb_status.
+//
+// Reference host code replaced (src/goldbach.cu): run_gpu_worker :378-500,
+// validate_hardware_and_limits :505-541, table builds in main :661-676.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "goldbach_gpu.h"
+#include "gb_kernels.hpp"
+
+using namespace gbk;
+
+namespace {
+
+constexpr uint64_t DEFAULT_SEG = 1ull << 27;
+constexpr uint64_t DEFAULT_P2 = 100000000ull;           // goldbach.cu:316
+constexpr uint64_t VRAM_MARGIN = 50ull << 20;            // goldbach.cu:96
+constexpr uint32_t PLAIN_TILE_BITS = 1u << 18;          // plain sieve tile (bits per workgroup)
+constexpr uint32_t MARK_WPB = 2048;                      // plain marking: verified words per workgroup
+constexpr uint32_t DEFAULT_TILE_LOG2 = 20;               // fused tile: 2^20 evens
+constexpr uint32_t SLOW_CAP = 1u << 20;
+constexpr uint32_t SEED_LIMIT = 4095;                    // trial-division seed (covers sqrt(2^22))
+
+std::mutex g_open_mu;
+std::string g_open_err;
+
+struct ItemTable {
+    uint32_t tb = 0;        // tile bits the schedule was cut for
+    uint32_t n_med = 0;
+    uint32_t* d_items = nullptr;
+    uint32_t* d_start = nullptr;
+};
+
+}  // namespace
+
+struct gb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    gb_config cfg{};
+    int cus = 0;
+    uint64_t total_mem = 0;
+    // prime tables
+    uint32_t* d_sp = nullptr;
+    double* d_inv = nullptr;
+    uint32_t* d_c32 = nullptr;
+    uint32_t nsp = 0;
+    std::vector<uint32_t> h_sp;
+    uint64_t* d_p2bits = nullptr;
+    uint64_t p2_limit = 0;   // odd
+    std::vector<ItemTable> items;
+    // plain segment buffers
+    uint64_t* d_window = nullptr;
+    uint64_t window_cap = 0;   // words
+    uint64_t* d_verified = nullptr;
+    uint64_t verified_cap = 0; // words
+    uint32_t* d_pmin = nullptr;
+    uint64_t pmin_cap = 0;
+    StatRec* d_rec = nullptr;
+    uint64_t rec_cap = 0;
+    StatRec* d_out = nullptr;
+    uint32_t out_cap = 0;
+    // slow list
+    uint64_t* d_slow_n = nullptr;
+    uint32_t* d_slow_win = nullptr;
+    uint64_t* d_slow_ctl = nullptr;   // [0]: count (u32) | overflow (u32) as two words
+    uint64_t* d_slow_res = nullptr;
+    // windows + reduce ranges
+    WinDesc* d_wins = nullptr;
+    uint64_t* d_rec_lo = nullptr;
+    uint64_t* d_rec_hi = nullptr;
+    uint32_t wins_cap = 0;
+    uint64_t* d_counter = nullptr;
+    // serving buffers
+    uint64_t* d_qn = nullptr;
+    uint64_t* d_qp = nullptr;
+    uint32_t q_cap = 0;
+    // host staging (pinned)
+    StatRec* h_out = nullptr;
+    uint64_t* h_ctl = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // cached batched graph
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graph_key;
+};
+
+namespace {
+
+int fail(gb_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HCK(ctx, call)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(ctx, GB_EDEVICE, std::string("HIP error at ") + __FILE__ + ":" +       \
+                                             std::to_string(__LINE__) + " -- " + hipGetErrorString(e_)); \
+    } while (0)
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+void dhostfree(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
+template <typename T>
+hipError_t dalloc(T** p, uint64_t count) {
+    *p = nullptr;
+    if (!count) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+uint32_t count_le(const std::vector<uint32_t>& sp, uint64_t x) {
+    return (uint32_t)(std::upper_bound(sp.begin(), sp.end(), (uint32_t)std::min<uint64_t>(x, 0xFFFFFFFFull)) - sp.begin());
+}
+
+// Medium-prime schedule for tiles of at most `tb` bits: prime k with
+// p*CHUNK < tb is cut into ceil(tb/(CHUNK*p)) items of CHUNK marks.
+int build_items(gb_ctx* c, uint32_t tb, ItemTable* out) {
+    for (auto& it : c->items)
+        if (it.tb == tb) { *out = it; return GB_OK; }
+    ItemTable t;
+    t.tb = tb;
+    std::vector<uint32_t> items, start;
+    uint32_t k = NPAT;
+    start.assign(NPAT + 1, 0);
+    while (k < c->nsp && (uint64_t)c->h_sp[k] * CHUNK < tb) {
+        const uint32_t p = c->h_sp[k];
+        const uint32_t chunks = (uint32_t)((tb + (uint64_t)CHUNK * p - 1) / ((uint64_t)CHUNK * p));
+        if (chunks > 4096) return fail(c, GB_EINVAL, "tile too large for the medium-prime schedule");
+        for (uint32_t ch = 0; ch < chunks; ++ch) items.push_back((k << 12) | ch);
+        ++k;
+        start.push_back((uint32_t)items.size());
+    }
+    t.n_med = k;
+    // start[k'] for k' in [0, n_med]: items of primes with index < k'
+    start.resize(t.n_med + 1);
+    HCK(c, dalloc(&t.d_items, items.size()));
+    HCK(c, dalloc(&t.d_start, start.size()));
+    if (!items.empty())
+        HCK(c, hipMemcpy(t.d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice));
+    HCK(c, hipMemcpy(t.d_start, start.data(), start.size() * 4, hipMemcpyHostToDevice));
+    c->items.push_back(t);
+    *out = t;
+    return GB_OK;
+}
+
+Tables make_tables(const gb_ctx* c, const ItemTable& it) {
+    Tables T;
+    T.sp = c->d_sp;
+    T.inv = c->d_inv;
+    T.c32 = c->d_c32;
+    T.items = it.d_items;
+    T.item_start = it.d_start;
+    T.p2bits = c->d_p2bits;
+    T.p2_limit = c->p2_limit;
+    T.cover = SP_MAX;
+    T.nsp = c->nsp;
+    T.n_med = it.n_med;
+    return T;
+}
+
+SlowList slow_list(const gb_ctx* c) {
+    SlowList s;
+    s.n = c->d_slow_n;
+    s.win = c->d_slow_win;
+    s.count = reinterpret_cast<uint32_t*>(c->d_slow_ctl);
+    s.overflow = reinterpret_cast<uint32_t*>(c->d_slow_ctl) + 2;
+    s.cap = SLOW_CAP;
+    s.pad = 0;
+    return s;
+}
+
+MarkParams mark_params(const gb_ctx* c, uint64_t p_small, uint32_t flags) {
+    MarkParams mp;
+    mp.p_small = p_small;
+    mp.bound = std::max(p_small, c->cfg.phase2_limit);
+    const uint64_t reach = std::min<uint64_t>(mp.bound, HALO_P);
+    mp.K = count_le(c->h_sp, reach);
+    mp.fast_final = mp.bound <= HALO_P ? 1u : 0u;
+    mp.flags = flags & GB_WANT_PMIN;
+    mp.pad = 0;
+    return mp;
+}
+
+void to_stats(const StatRec& r, double ms, gb_seg_stats* st) {
+    if (!st) return;
+    st->evens = r.evens;
+    st->phase2 = r.phase2;
+    st->failures = r.failures;
+    st->first_fail_n = r.failures ? r.first_fail : 0;
+    st->sum_pmin = r.sum_pmin;
+    st->max_pmin = r.max_pmin;
+    st->argmax_n = r.max_pmin ? r.argmax : 0;
+    st->kernel_ms = ms;
+}
+
+void add_stats(gb_seg_stats* a, const gb_seg_stats& b) {
+    a->evens += b.evens;
+    a->phase2 += b.phase2;
+    if (b.failures && (!a->failures || b.first_fail_n < a->first_fail_n)) a->first_fail_n = b.first_fail_n;
+    a->failures += b.failures;
+    a->sum_pmin += b.sum_pmin;
+    if (b.max_pmin > a->max_pmin || (b.max_pmin == a->max_pmin && b.max_pmin && b.argmax_n < a->argmax_n)) {
+        a->max_pmin = b.max_pmin;
+        a->argmax_n = b.argmax_n;
+    }
+    a->kernel_ms += b.kernel_ms;
+}
+
+int ensure_wins(gb_ctx* c, uint32_t n) {
+    if (n <= c->wins_cap) return GB_OK;
+    dfree(c->d_wins);
+    dfree(c->d_rec_lo);
+    dfree(c->d_rec_hi);
+    dfree(c->d_out);
+    HCK(c, dalloc(&c->d_wins, n));
+    HCK(c, dalloc(&c->d_rec_lo, n));
+    HCK(c, dalloc(&c->d_rec_hi, n));
+    HCK(c, dalloc(&c->d_out, n));
+    if (c->h_out) dhostfree(c->h_out);
+    HCK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_out), n * sizeof(StatRec), hipHostMallocDefault));
+    c->wins_cap = n;
+    return GB_OK;
+}
+
+int ensure_rec(gb_ctx* c, uint64_t n) {
+    if (n <= c->rec_cap) return GB_OK;
+    dfree(c->d_rec);
+    HCK(c, dalloc(&c->d_rec, n));
+    c->rec_cap = n;
+    return GB_OK;
+}
+
+// Build the prime tables on the device: trial-division seed to 4095, sieve of
+// [3, 2^22) with the seed, compaction, reciprocal tables, then the Phase-2
+// bitset of [3, phase2_limit] with the full table.
+int build_tables(gb_ctx* c) {
+    hipStream_t s = c->stream;
+    uint64_t* d_seedbits = nullptr;
+    uint32_t* d_seed = nullptr;
+    double* d_seed_inv = nullptr;
+    uint32_t* d_seed_c32 = nullptr;
+    uint32_t* d_cnt = nullptr;
+    const uint32_t seed_words = ((SEED_LIMIT - 3) / 2 + 1 + 63) / 64;
+    HCK(c, dalloc(&d_seedbits, seed_words));
+    HCK(c, dalloc(&d_seed, 1024));
+    HCK(c, dalloc(&d_seed_inv, 1024));
+    HCK(c, dalloc(&d_seed_c32, 1024));
+    HCK(c, dalloc(&d_cnt, 2));
+    HCK(c, launch_seed_primes(s, SEED_LIMIT, d_seedbits));
+    HCK(c, launch_compact_primes(s, d_seedbits, seed_words, SEED_LIMIT, d_seed, d_cnt));
+    uint32_t nseed = 0;
+    HCK(c, hipMemcpyAsync(&nseed, d_cnt, 4, hipMemcpyDeviceToHost, s));
+    HCK(c, hipStreamSynchronize(s));
+    HCK(c, launch_prime_aux(s, d_seed, nseed, d_seed_inv, d_seed_c32));
+
+    Tables seedT{};
+    seedT.sp = d_seed;
+    seedT.inv = d_seed_inv;
+    seedT.c32 = d_seed_c32;
+    seedT.items = nullptr;
+    seedT.item_start = nullptr;
+    seedT.p2bits = nullptr;
+    seedT.p2_limit = 0;
+    seedT.cover = SEED_LIMIT;
+    seedT.nsp = nseed;
+    seedT.n_med = NPAT;
+
+    const uint64_t sp_bits = (SP_MAX - 1 - 3) / 2 + 1;   // odd q in [3, 2^22 - 1]
+    const uint64_t sp_words = (sp_bits + 63) / 64;
+    uint64_t* d_spbits = nullptr;
+    HCK(c, dalloc(&d_spbits, sp_words));
+    HCK(c, launch_sieve_window(s, seedT, 3, 0, sp_bits, PLAIN_TILE_BITS, d_spbits, 0));
+    HCK(c, dalloc(&c->d_sp, 300000));
+    HCK(c, launch_compact_primes(s, d_spbits, (uint32_t)sp_words, SP_MAX - 1, c->d_sp, d_cnt));
+    HCK(c, hipMemcpyAsync(&c->nsp, d_cnt, 4, hipMemcpyDeviceToHost, s));
+    HCK(c, hipStreamSynchronize(s));
+    if (c->nsp == 0 || c->nsp > 300000) return fail(c, GB_EDEVICE, "prime table build produced a bad count");
+    HCK(c, dalloc(&c->d_inv, c->nsp));
+    HCK(c, dalloc(&c->d_c32, c->nsp));
+    HCK(c, launch_prime_aux(s, c->d_sp, c->nsp, c->d_inv, c->d_c32));
+    c->h_sp.resize(c->nsp);
+    HCK(c, hipMemcpyAsync(c->h_sp.data(), c->d_sp, c->nsp * 4ull, hipMemcpyDeviceToHost, s));
+    HCK(c, hipStreamSynchronize(s));
+    dfree(d_seedbits);
+    dfree(d_seed);
+    dfree(d_seed_inv);
+    dfree(d_seed_c32);
+    dfree(d_cnt);
+    dfree(d_spbits);
+
+    // Phase-2 bitset: odd primes <= phase2_limit
+    ItemTable it;
+    int rc = build_items(c, PLAIN_TILE_BITS, &it);
+    if (rc) return rc;
+    uint64_t lim = c->cfg.phase2_limit;
+    if (lim < 3) lim = 3;
+    if (!(lim & 1)) lim -= 1;
+    c->p2_limit = lim;
+    const uint64_t p2_bits = (lim - 3) / 2 + 1;
+    HCK(c, dalloc(&c->d_p2bits, (p2_bits + 63) / 64));
+    Tables T = make_tables(c, it);
+    T.p2bits = nullptr;
+    T.p2_limit = 0;
+    HCK(c, launch_sieve_window(s, T, 3, 0, p2_bits, PLAIN_TILE_BITS, c->d_p2bits, 0));
+    HCK(c, hipStreamSynchronize(s));
+    return GB_OK;
+}
+
+// One plain segment already validated; results left in c->h_out[0].
+int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool want_verified, bool want_pmin,
+                double* ms) {
+    hipStream_t s = c->stream;
+    ItemTable it;
+    int rc = build_items(c, PLAIN_TILE_BITS, &it);
+    if (rc) return rc;
+    const Tables T = make_tables(c, it);
+    const WinGeom g = window_geom(A, cnt + HALO + WIN_PAD);
+    const uint64_t wwords = ((uint64_t)g.nbits + 63) / 64;
+    const uint64_t vwords = (cnt + 63) / 64;
+    if (wwords > c->window_cap || vwords > c->verified_cap) return fail(c, GB_ERANGE, "segment larger than max_seg_evens");
+    if (want_pmin && cnt > c->pmin_cap) {
+        dfree(c->d_pmin);
+        c->d_pmin = nullptr;
+        c->pmin_cap = 0;
+        HCK(c, dalloc(&c->d_pmin, cnt));
+        c->pmin_cap = cnt;
+    }
+    const uint64_t nblocks = (vwords + MARK_WPB - 1) / MARK_WPB;
+    rc = ensure_rec(c, nblocks);
+    if (rc) return rc;
+    rc = ensure_wins(c, 1);
+    if (rc) return rc;
+    WinDesc wd{A, cnt, 0, 0};
+    uint64_t lohi[2] = {0, nblocks};
+    HCK(c, hipMemcpyAsync(c->d_wins, &wd, sizeof(wd), hipMemcpyHostToDevice, s));
+    HCK(c, hipMemcpyAsync(c->d_rec_lo, &lohi[0], 8, hipMemcpyHostToDevice, s));
+    HCK(c, hipMemcpyAsync(c->d_rec_hi, &lohi[1], 8, hipMemcpyHostToDevice, s));
+    SlowList sl = slow_list(c);
+    HCK(c, hipEventRecord(c->ev0, s));
+    HCK(c, launch_fill(s, c->d_slow_ctl, 2, 0));
+    HCK(c, launch_sieve_window(s, T, g.qv0, g.lead, g.nbits, PLAIN_TILE_BITS, c->d_window, 0));
+    uint32_t nrec = 0;
+    HCK(c, launch_mark_segment(s, T, mp, A, cnt, c->d_window, wwords, MARK_WPB,
+                               want_verified ? c->d_verified : nullptr, want_pmin ? c->d_pmin : nullptr, sl,
+                               c->d_rec, &nrec));
+    HCK(c, launch_slow(s, T, mp, sl, c->d_slow_res, c->d_wins, want_verified ? c->d_verified : nullptr,
+                       want_pmin ? c->d_pmin : nullptr, A));
+    HCK(c, launch_reduce(s, mp, c->d_rec, c->d_rec_lo, c->d_rec_hi, 1, sl, c->d_slow_res, c->d_out));
+    HCK(c, hipEventRecord(c->ev1, s));
+    HCK(c, hipMemcpyAsync(c->h_out, c->d_out, sizeof(StatRec), hipMemcpyDeviceToHost, s));
+    HCK(c, hipMemcpyAsync(c->h_ctl, c->d_slow_ctl, 16, hipMemcpyDeviceToHost, s));
+    HCK(c, hipStreamSynchronize(s));
+    float f = 0;
+    HCK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ms) *ms = f;
+    const uint32_t* ctl = reinterpret_cast<const uint32_t*>(c->h_ctl);
+    if (ctl[2]) return fail(c, GB_ERANGE, "slow-path list overflow (more than 2^20 n with p_min beyond the window)");
+    return GB_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* gb_last_error(const gb_ctx* ctx) {
+    if (!ctx) return g_open_err.c_str();
+    return ctx->err.c_str();
+}
+
+int gb_device_count(int* count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    if (count) *count = n;
+    return n > 0 ? GB_OK : GB_ENODEV;
+}
+
+int gb_device_info(int device, char* name, size_t name_len, uint64_t* total_mem, int* compute_units) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return GB_ENODEV;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return GB_EDEVICE;
+    if (name && name_len) {
+        const char* nm = p.name[0] ? p.name : p.gcnArchName;
+        std::snprintf(name, name_len, "%s", nm);
+    }
+    if (total_mem) *total_mem = p.totalGlobalMem;
+    if (compute_units) *compute_units = p.multiProcessorCount;
+    return GB_OK;
+}
+
+int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
+    std::lock_guard<std::mutex> lk(g_open_mu);
+    g_open_err.clear();
+    if (!out) { g_open_err = "gb_open: out is NULL"; return GB_EINVAL; }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_open_err = "No HIP devices found."; return GB_ENODEV; }
+    if (device < 0 || device >= ndev) { g_open_err = "device index out of range"; return GB_ENODEV; }
+    gb_ctx* c = new gb_ctx();
+    c->device = device;
+    if (cfg_in) c->cfg = *cfg_in;
+    if (!c->cfg.max_n) c->cfg.max_n = ~0ull - 1;
+    if (!c->cfg.max_seg_evens) c->cfg.max_seg_evens = DEFAULT_SEG;
+    if (!c->cfg.phase2_limit) c->cfg.phase2_limit = DEFAULT_P2;
+    if (!c->cfg.serve_slots) c->cfg.serve_slots = 65536;
+    auto bail = [&](int rc) {
+        g_open_err = c->err;
+        gb_close(c);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) { c->err = "hipSetDevice failed"; return bail(GB_EDEVICE); }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) { c->err = "hipGetDeviceProperties failed"; return bail(GB_EDEVICE); }
+    c->cus = p.multiProcessorCount;
+    c->total_mem = p.totalGlobalMem;
+    if (c->cfg.phase2_limit > (1ull << 36)) { c->err = "phase2_limit above 2^36"; return bail(GB_EINVAL); }
+
+    // device memory guard (validate_hardware_and_limits, goldbach.cu:505-541)
+    const uint64_t seg = c->cfg.max_seg_evens;
+    const uint64_t window_words = (seg + HALO + WIN_PAD + 63) / 64 + 1;
+    const uint64_t verified_words = (seg + 63) / 64;
+    const uint64_t p2_bytes = (c->cfg.phase2_limit / 2 / 64 + 2) * 8;
+    const uint64_t fixed = 300000ull * 16 + p2_bytes + SLOW_CAP * 28ull + (64ull << 20);
+    const uint64_t required = (window_words + verified_words) * 8 + fixed + VRAM_MARGIN;
+    const uint64_t budget = c->cfg.mem_budget ? std::min<uint64_t>(c->cfg.mem_budget, c->total_mem) : c->total_mem;
+    if (required > budget) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "insufficient VRAM: required %llu MB, available %llu MB",
+                      (unsigned long long)(required >> 20), (unsigned long long)(budget >> 20));
+        c->err = buf;
+        return bail(GB_ENOMEM);
+    }
+    if (hipStreamCreate(&c->stream) != hipSuccess) { c->err = "hipStreamCreate failed"; return bail(GB_EDEVICE); }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        c->err = "hipEventCreate failed";
+        return bail(GB_EDEVICE);
+    }
+    int rc = build_tables(c);
+    if (rc) return bail(rc);
+    if (dalloc(&c->d_window, window_words) != hipSuccess || dalloc(&c->d_verified, verified_words) != hipSuccess ||
+        dalloc(&c->d_slow_n, SLOW_CAP) != hipSuccess || dalloc(&c->d_slow_win, SLOW_CAP) != hipSuccess ||
+        dalloc(&c->d_slow_res, SLOW_CAP) != hipSuccess || dalloc(&c->d_slow_ctl, 2) != hipSuccess ||
+        dalloc(&c->d_counter, 1) != hipSuccess) {
+        c->err = "device allocation failed";
+        return bail(GB_ENOMEM);
+    }
+    c->window_cap = window_words;
+    c->verified_cap = verified_words;
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), 16, hipHostMallocDefault) != hipSuccess) {
+        c->err = "pinned allocation failed";
+        return bail(GB_ENOMEM);
+    }
+    rc = ensure_wins(c, 64);
+    if (rc) return bail(rc);
+    *out = c;
+    return GB_OK;
+}
+
+int gb_close(gb_ctx* c) {
+    if (!c) return GB_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (auto& it : c->items) {
+        dfree(it.d_items);
+        dfree(it.d_start);
+    }
+    void* bufs[] = {c->d_sp, c->d_inv, c->d_c32, c->d_p2bits, c->d_window, c->d_verified, c->d_pmin, c->d_rec,
+                    c->d_out, c->d_slow_n, c->d_slow_win, c->d_slow_ctl, c->d_slow_res, c->d_wins, c->d_rec_lo,
+                    c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp};
+    for (void* b : bufs)
+        if (b) dfree(b);
+    if (c->h_out) dhostfree(c->h_out);
+    if (c->h_ctl) dhostfree(c->h_ctl);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GB_OK;
+}
+
+int gb_sieve_window(gb_ctx* c, uint64_t q_low, uint64_t q_high, uint64_t* bits_out) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (!bits_out) return fail(c, GB_EINVAL, "bits_out is NULL");
+    if (q_low < 3 || !(q_low & 1) || !(q_high & 1) || q_high < q_low)
+        return fail(c, GB_EINVAL, "window must satisfy 3 <= q_low <= q_high, both odd");
+    HCK(c, hipSetDevice(c->device));
+    ItemTable it;
+    int rc = build_items(c, PLAIN_TILE_BITS, &it);
+    if (rc) return rc;
+    const Tables T = make_tables(c, it);
+    const uint64_t nbits = (q_high - q_low) / 2 + 1;
+    // stream the window through the segment buffer in word-aligned chunks
+    const uint64_t chunk_bits = (c->window_cap - 1) * 64;
+    for (uint64_t b0 = 0; b0 < nbits; b0 += chunk_bits) {
+        const uint64_t nb = std::min(chunk_bits, nbits - b0);
+        const uint64_t words = (nb + 63) / 64;
+        HCK(c, launch_sieve_window(c->stream, T, q_low + 2 * b0, 0, nb, PLAIN_TILE_BITS, c->d_window,
+                                   b0 + nb == nbits ? 1 : 0));
+        HCK(c, hipMemcpyAsync(bits_out + b0 / 64, c->d_window, words * 8, hipMemcpyDeviceToHost, c->stream));
+        HCK(c, hipStreamSynchronize(c->stream));
+    }
+    return GB_OK;
+}
+
+int gb_verify_segment(gb_ctx* c, uint64_t A, uint64_t cnt, uint64_t p_small, uint32_t flags, uint64_t* verified_out,
+                      uint32_t* pmin_out, gb_seg_stats* st) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (A < 4 || (A & 1)) return fail(c, GB_EINVAL, "seg_start must be even and >= 4");
+    if (cnt == 0) return fail(c, GB_EINVAL, "seg_even_count must be >= 1");
+    if (cnt - 1 > (~0ull - 1 - A) / 2) return fail(c, GB_EINVAL, "segment runs past 2^64-2");
+    if (cnt > c->cfg.max_seg_evens) return fail(c, GB_ERANGE, "seg_even_count above max_seg_evens");
+    HCK(c, hipSetDevice(c->device));
+    const MarkParams mp = mark_params(c, p_small, flags | (pmin_out ? GB_WANT_PMIN : 0));
+    double ms = 0;
+    int rc = run_segment(c, A, cnt, mp, verified_out != nullptr, pmin_out != nullptr, &ms);
+    if (rc) return rc;
+    if (verified_out)
+        HCK(c, hipMemcpy(verified_out, c->d_verified, ((cnt + 63) / 64) * 8, hipMemcpyDeviceToHost));
+    if (pmin_out) HCK(c, hipMemcpy(pmin_out, c->d_pmin, cnt * 4, hipMemcpyDeviceToHost));
+    to_stats(c->h_out[0], ms, st);
+    return GB_OK;
+}
+
+// Batched persistent form over windows; records land in c->h_out[0..nwin).
+static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t p_small, uint32_t flags,
+                       uint32_t tile_log2, uint32_t wg_per_cu, uint64_t* verified_dev, double* ms) {
+    hipStream_t s = c->stream;
+    if (tile_log2 == 0) tile_log2 = DEFAULT_TILE_LOG2;
+    if (tile_log2 < 12 || tile_log2 > 20) return fail(c, GB_EINVAL, "tile_log2 must be in [12, 20]");
+    const uint32_t te = 1u << tile_log2;
+    ItemTable it;
+    int rc = build_items(c, te + HALO + WIN_PAD, &it);
+    if (rc) return rc;
+    const Tables T = make_tables(c, it);
+    const MarkParams mp = mark_params(c, p_small, flags);
+    const uint32_t nwin = (uint32_t)wins_in.size();
+    std::vector<WinDesc> wins(wins_in);
+    std::vector<uint64_t> lo(nwin), hi(nwin);
+    uint64_t tiles = 0, words = 0;
+    for (uint32_t w = 0; w < nwin; ++w) {
+        wins[w].tile0 = tiles;
+        wins[w].word0 = words;
+        lo[w] = tiles;
+        tiles += (wins[w].evens + te - 1) / te;
+        hi[w] = tiles;
+        words += (wins[w].evens + 63) / 64;
+    }
+    rc = ensure_rec(c, tiles);
+    if (rc) return rc;
+    rc = ensure_wins(c, nwin);
+    if (rc) return rc;
+    HCK(c, hipMemcpyAsync(c->d_wins, wins.data(), nwin * sizeof(WinDesc), hipMemcpyHostToDevice, s));
+    HCK(c, hipMemcpyAsync(c->d_rec_lo, lo.data(), nwin * 8, hipMemcpyHostToDevice, s));
+    HCK(c, hipMemcpyAsync(c->d_rec_hi, hi.data(), nwin * 8, hipMemcpyHostToDevice, s));
+    if (wg_per_cu == 0) wg_per_cu = 1;
+    uint64_t grid = (uint64_t)c->cus * wg_per_cu;
+    if (grid > tiles) grid = tiles;
+    SlowList sl = slow_list(c);
+
+    std::vector<uint64_t> key;
+    for (auto& w : wins) { key.push_back(w.start); key.push_back(w.evens); }
+    key.push_back(p_small);
+    key.push_back(flags);
+    key.push_back(tile_log2);
+    key.push_back(grid);
+    key.push_back((uint64_t)(uintptr_t)verified_dev);
+    key.push_back((uint64_t)(uintptr_t)c->d_rec);
+    key.push_back((uint64_t)(uintptr_t)c->d_wins);
+    key.push_back((uint64_t)(uintptr_t)c->d_out);
+    if (!c->graph || key != c->graph_key) {
+        if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+        HCK(c, hipStreamSynchronize(s));
+        hipGraph_t g = nullptr;
+        HCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = launch_fill(s, c->d_slow_ctl, 2, 0);
+        if (e == hipSuccess) e = launch_fill(s, c->d_counter, 1, 0);
+        if (e == hipSuccess)
+            e = launch_fused(s, T, mp, c->d_wins, nwin, tiles, te, (uint32_t)grid, c->d_counter, verified_dev, sl,
+                             c->d_rec);
+        if (e == hipSuccess) e = launch_slow(s, T, mp, sl, c->d_slow_res, c->d_wins, verified_dev, nullptr, 0);
+        if (e == hipSuccess)
+            e = launch_reduce(s, mp, c->d_rec, c->d_rec_lo, c->d_rec_hi, nwin, sl, c->d_slow_res, c->d_out);
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); HCK(c, e); }
+        HCK(c, e2);
+        hipError_t e3 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HCK(c, e3);
+        c->graph_key = key;
+    }
+    HCK(c, hipEventRecord(c->ev0, s));
+    HCK(c, hipGraphLaunch(c->graph, s));
+    HCK(c, hipEventRecord(c->ev1, s));
+    HCK(c, hipMemcpyAsync(c->h_out, c->d_out, nwin * sizeof(StatRec), hipMemcpyDeviceToHost, s));
+    HCK(c, hipMemcpyAsync(c->h_ctl, c->d_slow_ctl, 16, hipMemcpyDeviceToHost, s));
+    HCK(c, hipStreamSynchronize(s));
+    float f = 0;
+    HCK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ms) *ms = f;
+    const uint32_t* ctl = reinterpret_cast<const uint32_t*>(c->h_ctl);
+    if (ctl[2]) return fail(c, GB_ERANGE, "slow-path list overflow (more than 2^20 n with p_min beyond the window)");
+    return GB_OK;
+}
+
+int gb_verify_range(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, const gb_tuning* tun,
+                    gb_seg_stats* st, volatile uint64_t* progress) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (limit & 1) limit--;             // goldbach.cu:594
+    if (start < 4) start = 4;           // goldbach.cu:604-605
+    if (start & 1) start++;
+    if (limit < 4) return fail(c, GB_EINVAL, "LIMIT must be >= 4");
+    if (start > limit) return fail(c, GB_EINVAL, "START must not exceed LIMIT");
+    HCK(c, hipSetDevice(c->device));
+    gb_tuning t{};
+    if (tun) t = *tun;
+    const uint64_t evens = (limit - start) / 2 + 1;
+    gb_seg_stats tot{};
+    if (t.form == GB_FORM_BATCHED) {
+        std::vector<WinDesc> w(1);
+        w[0] = WinDesc{start, evens, 0, 0};
+        double ms = 0;
+        int rc = run_batched(c, w, p_small, t.flags, t.tile_log2, t.wg_per_cu, nullptr, &ms);
+        if (rc) return rc;
+        to_stats(c->h_out[0], ms, &tot);
+        if (progress) *progress = evens;
+        if (st) *st = tot;
+        return GB_OK;
+    }
+    if (t.form != GB_FORM_PLAIN) return fail(c, GB_EINVAL, "unknown form");
+    uint64_t seg = t.seg_evens ? t.seg_evens : std::min<uint64_t>(DEFAULT_SEG, c->cfg.max_seg_evens);
+    if (seg > c->cfg.max_seg_evens) return fail(c, GB_ERANGE, "seg_evens above max_seg_evens");
+    const MarkParams mp = mark_params(c, p_small, t.flags);
+    uint64_t done = 0;
+    for (uint64_t a = start;;) {
+        const uint64_t left = (limit - a) / 2 + 1;
+        const uint64_t cnt = std::min(left, seg);
+        double ms = 0;
+        int rc = run_segment(c, a, cnt, mp, true, false, &ms);   // verified map written, as the reference does
+        if (rc) return rc;
+        gb_seg_stats one{};
+        to_stats(c->h_out[0], ms, &one);
+        add_stats(&tot, one);
+        done += cnt;
+        if (progress) *progress = done;
+        if (one.failures) break;        // the reference stops at the first failure (goldbach.cu:420,476-480)
+        if (cnt == left) break;
+        a += 2 * cnt;
+    }
+    if (st) *st = tot;
+    return GB_OK;
+}
+
+int gb_verify_windows(gb_ctx* c, const gb_window* windows, uint32_t count, uint64_t p_small, uint32_t flags,
+                      gb_seg_stats* per_window, gb_seg_stats* total) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (!windows || count == 0) return fail(c, GB_EINVAL, "no windows");
+    std::vector<WinDesc> w(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint64_t a = windows[i].start, e = windows[i].evens;
+        if (a < 4 || (a & 1) || e == 0 || e - 1 > (~0ull - 1 - a) / 2)
+            return fail(c, GB_EINVAL, "window " + std::to_string(i) + " outside [4, 2^64-2] or not even");
+        w[i] = WinDesc{a, e, 0, 0};
+    }
+    HCK(c, hipSetDevice(c->device));
+    double ms = 0;
+    int rc = run_batched(c, w, p_small, flags, 0, 0, nullptr, &ms);
+    if (rc) return rc;
+    gb_seg_stats tot{};
+    for (uint32_t i = 0; i < count; ++i) {
+        gb_seg_stats one{};
+        to_stats(c->h_out[i], 0, &one);
+        if (per_window) per_window[i] = one;
+        add_stats(&tot, one);
+    }
+    tot.kernel_ms = ms;
+    if (total) *total = tot;
+    return GB_OK;
+}
+
+int gb_min_partition_batch(gb_ctx* c, const uint64_t* n, uint32_t count, uint64_t* p_out) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (count == 0) return GB_OK;
+    if (!n || !p_out) return fail(c, GB_EINVAL, "NULL buffer");
+    HCK(c, hipSetDevice(c->device));
+    if (count > c->q_cap) {
+        dfree(c->d_qn);
+        dfree(c->d_qp);
+        c->d_qn = c->d_qp = nullptr;
+        c->q_cap = 0;
+        HCK(c, dalloc(&c->d_qn, count));
+        HCK(c, dalloc(&c->d_qp, count));
+        c->q_cap = count;
+    }
+    ItemTable it;
+    int rc = build_items(c, PLAIN_TILE_BITS, &it);
+    if (rc) return rc;
+    const Tables T = make_tables(c, it);
+    HCK(c, hipMemcpyAsync(c->d_qn, n, count * 8ull, hipMemcpyHostToDevice, c->stream));
+    HCK(c, launch_min_partition(c->stream, T, c->d_qn, count, c->cfg.phase2_limit, c->d_qp));
+    HCK(c, hipMemcpyAsync(p_out, c->d_qp, count * 8ull, hipMemcpyDeviceToHost, c->stream));
+    HCK(c, hipStreamSynchronize(c->stream));
+    return GB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// gb_device.hpp -- device-side building blocks for gfx950 (wave64, 160 KB LDS):
+// exact small-modulus reduction, 64-bit Montgomery Miller-Rabin, the word
+// funnel shift used by the partition marker, and the geometry shared by the
+// sieve and marking kernels.
+//
+// Everything here is integer-exact.  The fp64 paths only ever hold integers
+// below 2^53, where fp64 arithmetic is exact, and every quotient estimate is
+// corrected by one compare.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gbk {
+
+// ---------------------------------------------------------------------------
+// Geometry
+// ---------------------------------------------------------------------------
+// Every window (fused tile or plain segment) starting at even n0 holds the
+// odd q = n0 - 2*HALO - 1 + 2i at bit i, so q = n - p for n = n0 + 2j sits at
+// bit HALO + j - (p-1)/2.  Primes p <= 2*HALO+1 are answered from the window.
+constexpr uint32_t HALO = 4096;              // odds below the first n
+constexpr uint32_t HALO_P = 2 * HALO + 1;    // largest p served by the window: 8193
+constexpr uint32_t WIN_PAD = 64;             // extra window bits read by the last word
+constexpr uint32_t SP_MAX = 1u << 22;        // sieving-prime table bound (odd primes <= 2^22)
+constexpr int NPAT = 17;                     // odd primes 3..61 applied as word patterns
+constexpr uint32_t CHUNK = 32;               // marks per medium-prime work item
+constexpr uint32_t MR_PRESIEVE = 1u << 18;   // pre-sieve bound for tiles decided by Miller-Rabin
+
+struct Tables {
+    const uint32_t* sp;     // odd primes <= SP_MAX ascending (sp[0] = 3, sp[17] = 67)
+    const double* inv;      // 1.0 / p (rounded)
+    const uint32_t* c32;    // 2^32 mod p
+    const uint32_t* items;  // medium-prime work items: (prime index << 12) | chunk
+    const uint32_t* item_start;  // item_start[k] = first item of medium prime k
+    const uint64_t* p2bits; // odd-only bitset of primes <= p2_limit (bit (q-3)/2)
+    uint64_t p2_limit;      // odd upper end of p2bits
+    uint64_t cover;         // sp holds every odd prime <= cover
+    uint32_t nsp;           // entries in sp
+    uint32_t n_med;         // medium primes: sp[17 .. n_med) are split into items
+};
+
+// Window geometry for a window starting at even n0 and covering `want` bits.
+struct WinGeom {
+    uint64_t qv0;   // q at bit `lead` (odd, >= 3)
+    uint32_t lead;  // bits [0, lead) stand for q < 3 and are zero
+    uint32_t nbits; // materialised bits (clipped so that q <= 2^64-1)
+};
+
+__host__ __device__ inline WinGeom window_geom(uint64_t n0, uint64_t want) {
+    WinGeom g;
+    if (n0 >= 2ull * HALO + 4) {
+        g.qv0 = n0 - 2ull * HALO - 1;
+        g.lead = 0;
+    } else {
+        g.lead = (uint32_t)(HALO + 2 - n0 / 2);
+        g.qv0 = 3;
+    }
+    // last q = qv0 + 2(nbits - 1 - lead) must stay <= 2^64 - 1
+    uint64_t room = (~0ull - g.qv0) / 2 + 1 + g.lead;
+    g.nbits = (uint32_t)(want < room ? want : room);
+    return g;
+}
+
+__host__ __device__ inline uint64_t geom_q_top(const WinGeom& g) {
+    return g.qv0 + 2ull * (g.nbits - 1 - g.lead);
+}
+
+// ---------------------------------------------------------------------------
+// Exact arithmetic helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t isqrt64(uint64_t x) {
+    uint64_t r = (uint64_t)sqrt((double)x);
+    if (r > 0xFFFFFFFFull) r = 0xFFFFFFFFull;
+    while (r * r > x) --r;
+    while (r < 0xFFFFFFFFull && (r + 1) * (r + 1) <= x) ++r;
+    return r;
+}
+
+// x mod p for odd p < 2^22 (inv = 1/p, c32 = 2^32 mod p).  Two fp64 steps,
+// each on an integer below 2^53, each corrected by one compare.
+__device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t p, double inv, uint32_t c32) {
+    const double pd = (double)p;
+    double d = (double)(uint32_t)(x >> 32);
+    double q = floor(d * inv);
+    double r = fma(-q, pd, d);
+    r = r < 0.0 ? r + pd : (r >= pd ? r - pd : r);
+    double y = fma(r, (double)c32, (double)(uint32_t)x);
+    q = floor(y * inv);
+    r = fma(-q, pd, y);
+    r = r < 0.0 ? r + pd : (r >= pd ? r - pd : r);
+    return (uint32_t)r;
+}
+
+// First bit index >= lb holding an odd multiple m of p with m >= p^2, where
+// bit lb holds the odd q_lo.  Returns an index; the caller bounds it.
+__device__ __forceinline__ uint64_t first_mark(uint64_t q_lo, uint32_t lb, uint32_t p, double inv, uint32_t c32) {
+    uint32_t r = mod_small(q_lo, p, inv, c32);
+    uint64_t d = r ? (uint64_t)(p - r) : 0;
+    if (d & 1) d += p;                 // q_lo + d odd
+    uint64_t pp = (uint64_t)p * p;
+    if (q_lo < pp && q_lo + d < pp) return lb + (pp - q_lo) / 2;
+    return lb + d / 2;
+}
+
+// 64-bit funnel: bits [s, s+64) of the 128-bit value hi:lo, s uniform in [0, 64).
+__device__ __forceinline__ uint64_t funnel64(uint64_t lo, uint64_t hi, uint32_t s) {
+    uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32);
+    uint32_t h0 = (uint32_t)hi, h1 = (uint32_t)(hi >> 32);
+    uint32_t r0, r1;
+    if (s < 32) {
+        r0 = __builtin_amdgcn_alignbit(l1, l0, s);
+        r1 = __builtin_amdgcn_alignbit(h0, l1, s);
+    } else {
+        r0 = __builtin_amdgcn_alignbit(h0, l1, s - 32);
+        r1 = __builtin_amdgcn_alignbit(h1, h0, s - 32);
+    }
+    return ((uint64_t)r1 << 32) | r0;
+}
+
+// ---------------------------------------------------------------------------
+// Montgomery arithmetic modulo odd n < 2^64, R = 2^64 (Montgomery 1985).
+// REDC in subtractive form: with m = lo * n^{-1} mod R, (ab - mn)/R = hi - mulhi(m, n)
+// lies in (-n, n), so one conditional add of n finishes the reduction and no
+// 65-bit intermediate ever appears.
+// ---------------------------------------------------------------------------
+struct Mont {
+    uint64_t n;
+    uint64_t ninv;  // n^{-1} mod 2^64
+    uint64_t one;   // R mod n
+};
+
+__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, const Mont& M) {
+    uint64_t lo = a * b;
+    uint64_t hi = __umul64hi(a, b);
+    uint64_t m = lo * M.ninv;
+    uint64_t mh = __umul64hi(m, M.n);
+    uint64_t t = hi - mh;
+    return hi < mh ? t + M.n : t;
+}
+
+__device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t n) {
+    // a, b < n: (a + b) mod n without 65-bit overflow
+    uint64_t s = a + b;
+    return (s < a || s >= n) ? s - n : s;
+}
+
+__device__ __forceinline__ Mont mont_setup(uint64_t n) {
+    Mont M;
+    M.n = n;
+    uint64_t x = n;                    // n*n == 1 mod 8 for odd n: 3 correct bits
+    for (int i = 0; i < 5; ++i) x *= 2 - n * x;   // 6, 12, 24, 48, 96 bits
+    M.ninv = x;
+    M.one = (0ull - n) % n;            // 2^64 mod n
+    return M;
+}
+
+// Strong probable-prime test to base a (a < n), n odd > 2, n - 1 = d * 2^r.
+__device__ __forceinline__ bool sprp(uint64_t a, uint64_t d, uint32_t r, const Mont& M) {
+    // a in Montgomery form: a * R mod n by double-and-add (a <= 37)
+    uint64_t am = 0;
+    for (int b = 5; b >= 0; --b) {
+        am = add_mod(am, am, M.n);
+        if ((a >> b) & 1) am = add_mod(am, M.one, M.n);
+    }
+    uint64_t x = am;
+    int top = 63 - __clzll((long long)d);
+    for (int b = top - 1; b >= 0; --b) {
+        x = mont_mul(x, x, M);
+        if ((d >> b) & 1) x = mont_mul(x, am, M);
+    }
+    const uint64_t minus_one = M.n - M.one;
+    if (x == M.one || x == minus_one) return true;
+    for (uint32_t i = 1; i < r; ++i) {
+        x = mont_mul(x, x, M);
+        if (x == minus_one) return true;
+        if (x == M.one) return false;
+    }
+    return false;
+}
+
+// Deterministic for every 64-bit n (bases 2..37; Sorenson & Webster 2015).
+// Small n (< 41) are answered directly; n even is composite.
+__device__ inline bool is_prime_u64(uint64_t n) {
+    if (n < 41) return (0x20A08A28ACull >> n) & 1;   // primes 2..37 as a mask
+    if ((n & 1) == 0) return false;
+    if (n % 3 == 0 || n % 5 == 0 || n % 7 == 0) return false;
+    Mont M = mont_setup(n);
+    uint64_t d = n - 1;
+    uint32_t r = __builtin_ctzll(d);
+    d >>= r;
+    const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+#pragma unroll 1
+    for (int i = 0; i < 12; ++i)
+        if (!sprp(bases[i], d, r, M)) return false;
+    return true;
+}
+
+// Exact primality using the resident p2bits table when q is small enough.
+__device__ __forceinline__ bool is_prime_q(uint64_t q, const Tables& T) {
+    if (q <= T.p2_limit) {
+        if (q < 3) return q == 2;
+        if ((q & 1) == 0) return false;
+        uint64_t b = (q - 3) >> 1;
+        return (T.p2bits[b >> 6] >> (b & 63)) & 1;
+    }
+    return is_prime_u64(q);
+}
+
+// Per-block statistics record, merged by the reduce kernel.
+struct StatRec {
+    uint64_t phase2;
+    uint64_t failures;
+    uint64_t first_fail;   // UINT64_MAX when none
+    uint64_t sum_pmin;
+    uint64_t max_pmin;
+    uint64_t argmax;       // UINT64_MAX when none
+    uint64_t evens;
+    uint64_t pad;
+};
+
+}  // namespace gbk

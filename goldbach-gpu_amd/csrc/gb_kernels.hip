@@ -1,0 +1,754 @@
+// gb_kernels.hip -- hand-written gfx950 kernels for the segmented double-sieve
+// Goldbach verifier.
+//
+//   sieve_tile      odd-only prime bitset of one window tile in LDS:
+//                   word patterns for p <= 61, LDS atomics for 67 <= p,
+//                   medium primes split into 32-mark work items so a wave's
+//                   lanes stride p 32-bit words apart (distinct LDS banks),
+//                   Miller-Rabin on pre-sieve survivors above 2^44.
+//   mark_words      Phase 1 + in-window Phase 2: a lane owns 64 consecutive
+//                   even n; for each odd prime p (wave-uniform) the 64 q = n-p
+//                   are 64 consecutive window bits, so one funnel shift + OR
+//                   marks 64 n.  Newly set bits give p_min exactly.
+//   fused_kernel    persistent workgroups pull tiles from a device counter and
+//                   run sieve_tile -> mark_words entirely in LDS (no HBM round
+//                   trip of the window).
+//   slow_kernel     the rare n whose p_min lies above the window's reach:
+//                   primes from the table, q by bitset or Miller-Rabin.
+//
+// Reference semantics restated: goldbach.cu:160-179 (q resolution), :198-259
+// (segment sieve), :288-311 (Phase 1), :362-373 (Phase 2), :184-195 (count).
+#include "gb_kernels.hpp"
+
+namespace gbk {
+
+// ---------------------------------------------------------------------------
+// constants
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr uint32_t pat_p(int i) {
+    constexpr uint32_t P[NPAT] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61};
+    return P[i];
+}
+
+// bits 0, p, 2p, ... below 64
+__host__ __device__ constexpr uint64_t pat_mask(uint32_t p) {
+    uint64_t m = 0;
+    for (uint32_t b = 0; b < 64; b += p) m |= 1ull << b;
+    return m;
+}
+
+__device__ __forceinline__ uint32_t count_le(const uint32_t* sp, uint32_t n, uint64_t x) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)sp[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __noinline__ bool is_prime_u64_call(uint64_t q) { return is_prime_u64(q); }
+
+__device__ __forceinline__ void lds_clear_bit(uint32_t* win32, uint32_t i) {
+    atomicAnd(&win32[i >> 5], ~(1u << (i & 31)));
+}
+
+// ---------------------------------------------------------------------------
+// sieve_tile: window bits [g0, g0 + nbits) of a window whose bit i (>= lead)
+// holds q = qv0 + 2(i - lead).  Result in win32[0 .. ceil(nbits/32)).
+// ---------------------------------------------------------------------------
+template <int BS>
+__device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ med_i0, uint64_t qv0,
+                           uint32_t lead, uint64_t g0, uint32_t nbits, const Tables& T, uint32_t tid) {
+    uint64_t* win64 = reinterpret_cast<uint64_t*>(win32);
+    const uint32_t nw64 = (nbits + 63) >> 6;
+    const uint64_t b_lo = g0 > lead ? g0 : (uint64_t)lead;
+    const bool any_valid = b_lo < g0 + nbits;
+    const uint32_t lb = any_valid ? (uint32_t)(b_lo - g0) : nbits;
+    const uint64_t q_lo = any_valid ? qv0 + 2ull * (b_lo - lead) : 3;
+    const uint64_t q_top = any_valid ? q_lo + 2ull * (nbits - 1 - lb) : 3;
+    const uint64_t s = isqrt64(q_top);
+    const bool mr = s > T.cover;
+    const uint32_t np = count_le(T.sp, T.nsp, mr ? (uint64_t)MR_PRESIEVE : s);
+
+    // 1) primes 3..61 as per-word masks: bit j of word k is q(64k+j); the
+    //    multiples of p in a word sit at j0, j0+p, ... with j0 stepping down by
+    //    64*BS mod p between this thread's words.
+    uint32_t j0[NPAT];
+#pragma unroll
+    for (int i = 0; i < NPAT; ++i) {
+        const uint32_t p = pat_p(i);
+        const uint32_t r0 = (uint32_t)((q_lo % p + p - (2ull * lb) % p) % p);   // q at local bit 0, mod p
+        const uint32_t res = (r0 + (128u * tid) % p) % p;
+        const uint32_t neg = res ? p - res : 0;
+        j0[i] = (neg * ((p + 1) / 2)) % p;
+    }
+    for (uint32_t k = tid; k < nw64; k += BS) {
+        uint64_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < NPAT; ++i) {
+            const uint32_t p = pat_p(i);
+            acc |= pat_mask(p) << j0[i];
+            const int t = (int)j0[i] - (int)((64u * BS) % p);
+            j0[i] = t < 0 ? (uint32_t)(t + (int)p) : (uint32_t)t;
+        }
+        uint64_t w = ~acc;
+        const uint64_t gb = g0 + 64ull * k;
+        if (gb < lead) {
+            const uint64_t zb = lead - gb;
+            w = zb >= 64 ? 0 : (w & (~0ull << zb));
+        }
+        if (any_valid && qv0 <= 61) {
+            // the pattern primes are themselves prime
+#pragma unroll
+            for (int i = 0; i < NPAT; ++i) {
+                const uint32_t p = pat_p(i);
+                if (p >= qv0) {
+                    const uint64_t bi = lead + (p - qv0) / 2;
+                    if (bi >= gb && bi < gb + 64) w |= 1ull << (bi - gb);
+                }
+            }
+        }
+        win64[k] = w;
+    }
+
+    // 2) first marks of the medium primes (split into CHUNK-mark items)
+    const uint32_t kmed_end = np < T.n_med ? np : T.n_med;
+    for (uint32_t k = NPAT + tid; k < kmed_end; k += BS) {
+        uint64_t i0 = any_valid ? first_mark(q_lo, lb, T.sp[k], T.inv[k], T.c32[k]) : nbits;
+        med_i0[k - NPAT] = i0 < nbits ? (uint32_t)i0 : nbits;
+    }
+    __syncthreads();
+
+    // 3) marks: medium items, then one thread per large prime
+    const uint32_t n_items = kmed_end > NPAT ? T.item_start[kmed_end] : 0;
+    const uint32_t n_large = np > T.n_med ? np - T.n_med : 0;
+    const uint32_t total = n_items + n_large;
+    for (uint32_t u = tid; u < total; u += BS) {
+        if (u < n_items) {
+            const uint32_t it = T.items[u];
+            const uint32_t k = it >> 12, c = it & 4095u;
+            const uint32_t p = T.sp[k];
+            uint32_t i = med_i0[k - NPAT] + c * (CHUNK * p);
+            const uint32_t m_end = i < nbits ? (nbits - 1 - i) / p + 1 : 0;
+            const uint32_t m_cnt = m_end < CHUNK ? m_end : CHUNK;
+            for (uint32_t m = 0; m < m_cnt; ++m, i += p) lds_clear_bit(win32, i);
+        } else if (any_valid) {
+            const uint32_t k = T.n_med + (u - n_items);
+            const uint32_t p = T.sp[k];
+            const uint64_t i0 = first_mark(q_lo, lb, p, T.inv[k], T.c32[k]);
+            if (i0 < nbits)
+                for (uint32_t i = (uint32_t)i0; i < nbits; i += p) lds_clear_bit(win32, i);
+        }
+    }
+    __syncthreads();
+
+    // 4) above 2^44 the table cannot reach sqrt(q): survivors of the pre-sieve
+    //    are decided by deterministic Miller-Rabin.
+    if (mr) {
+        for (uint32_t k = tid; k < nw64; k += BS) {
+            uint64_t w = win64[k], keep = w;
+            const uint64_t gb = g0 + 64ull * k;
+            while (w) {
+                const uint32_t b = __builtin_ctzll(w);
+                w &= w - 1;
+                const uint64_t gi = gb + b;
+                if (gi >= g0 + nbits) break;
+                if (!is_prime_u64_call(qv0 + 2ull * (gi - lead))) keep &= ~(1ull << b);
+            }
+            win64[k] = keep;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// statistics
+// ---------------------------------------------------------------------------
+struct LaneStats {
+    uint64_t ph2, fail, first, sum, mx, arg;
+};
+
+__device__ __forceinline__ LaneStats lane_stats_zero() {
+    LaneStats s;
+    s.ph2 = 0;
+    s.fail = 0;
+    s.first = ~0ull;
+    s.sum = 0;
+    s.mx = 0;
+    s.arg = ~0ull;
+    return s;
+}
+
+__device__ __forceinline__ void merge_max(uint64_t& mx, uint64_t& arg, uint64_t m2, uint64_t a2) {
+    if (m2 > mx || (m2 == mx && a2 < arg)) { mx = m2; arg = a2; }
+}
+
+__device__ __forceinline__ void wave_reduce(LaneStats& s) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        s.ph2 += __shfl_xor(s.ph2, o);
+        s.fail += __shfl_xor(s.fail, o);
+        const uint64_t f = __shfl_xor(s.first, o);
+        s.first = f < s.first ? f : s.first;
+        s.sum += __shfl_xor(s.sum, o);
+        const uint64_t m2 = __shfl_xor(s.mx, o), a2 = __shfl_xor(s.arg, o);
+        merge_max(s.mx, s.arg, m2, a2);
+    }
+}
+
+// Block-wide merge of lane statistics into one record (all threads must call).
+template <int BS>
+__device__ void block_store(LaneStats s, uint64_t evens, StatRec* out, uint32_t tid) {
+    __shared__ LaneStats red[BS / 64];
+    wave_reduce(s);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        LaneStats t = red[0];
+        for (int w = 1; w < BS / 64; ++w) {
+            const LaneStats& u = red[w];
+            t.ph2 += u.ph2;
+            t.fail += u.fail;
+            t.first = u.first < t.first ? u.first : t.first;
+            t.sum += u.sum;
+            merge_max(t.mx, t.arg, u.mx, u.arg);
+        }
+        StatRec r;
+        r.phase2 = t.ph2;
+        r.failures = t.fail;
+        r.first_fail = t.first;
+        r.sum_pmin = t.sum;
+        r.max_pmin = t.mx;
+        r.argmax = t.arg;
+        r.evens = evens;
+        r.pad = 0;
+        *out = r;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// mark_words: evens n = n_first + 2j, j in [0, cnt).  win64 bit (HALO + j - (p-1)/2)
+// holds q = n - p.  Lanes own 64-even words; the prime loop is wave-uniform.
+// ---------------------------------------------------------------------------
+template <int BS>
+__device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first, uint32_t cnt,
+                           const MarkParams& mp, const uint32_t* __restrict__ sp, uint64_t* __restrict__ verified,
+                           uint32_t* __restrict__ pmin_out, const SlowList& sl, uint32_t win_idx, LaneStats& st,
+                           uint32_t tid) {
+    const uint32_t lane = tid & 63, wave = tid >> 6;
+    const uint32_t nw = (cnt + 63) >> 6;
+    const bool want = mp.flags & 1u;
+    for (uint32_t base = wave * 64; base < nw; base += BS) {
+        const uint32_t w = base + lane;
+        const bool act = w < nw;
+        const uint32_t wr = act ? w : 0;
+        uint64_t valid = 0;
+        if (act) valid = (w == nw - 1 && (cnt & 63)) ? ((1ull << (cnt & 63)) - 1) : ~0ull;
+        uint64_t v = ~valid;
+        const bool has4 = act && w == 0 && n_first == 4;
+        if (has4 && mp.bound >= 2) {
+            v |= 1;   // 4 = 2 + 2, the only partition with p = 2
+            if (want) {
+                st.sum += 2;
+                merge_max(st.mx, st.arg, 2, 4);
+                if (pmin_out) pmin_out[0] = 2;
+            }
+        }
+        uint64_t v1 = ~valid | ((has4 && mp.p_small >= 2) ? 1ull : 0ull);
+        bool crossed = mp.p_small < 3;
+        for (uint32_t k = 0; k < mp.K; ++k) {
+            const uint32_t p = sp[k];
+            if (!crossed && p > mp.p_small) {
+                v1 = v;
+                crossed = true;
+            }
+            const uint32_t off = HALO - ((p - 1) >> 1);
+            const uint32_t g = off >> 6, s = off & 63;
+            const uint64_t R = funnel64(win64[wr + g], win64[wr + g + 1], s);
+            if (want) {
+                uint64_t nb = R & ~v;
+                if (nb) {
+                    st.sum += (uint64_t)__popcll(nb) * p;
+                    merge_max(st.mx, st.arg, p, n_first + 2ull * (64ull * w + __builtin_ctzll(nb)));
+                    if (pmin_out)
+                        while (nb) {
+                            const uint32_t b = __builtin_ctzll(nb);
+                            nb &= nb - 1;
+                            pmin_out[64u * w + b] = p;
+                        }
+                }
+            }
+            v |= R;
+            if ((k & 7) == 7 && __all(v == ~0ull)) break;
+        }
+        if (!crossed) v1 = v;
+        if (!act) continue;
+        st.ph2 += __popcll(~v1 & valid);
+        uint64_t u = ~v & valid;
+        if (u) {
+            if (mp.fast_final) {
+                st.fail += __popcll(u);
+                const uint64_t n = n_first + 2ull * (64ull * w + __builtin_ctzll(u));
+                st.first = n < st.first ? n : st.first;
+                if (pmin_out)
+                    while (u) {
+                        const uint32_t b = __builtin_ctzll(u);
+                        u &= u - 1;
+                        pmin_out[64u * w + b] = 0;
+                    }
+            } else {
+                while (u) {
+                    const uint32_t b = __builtin_ctzll(u);
+                    u &= u - 1;
+                    const uint32_t slot = atomicAdd(sl.count, 1u);
+                    if (slot < sl.cap) {
+                        sl.n[slot] = n_first + 2ull * (64ull * w + b);
+                        sl.win[slot] = win_idx;
+                    } else {
+                        atomicOr(sl.overflow, 1u);
+                    }
+                }
+            }
+        }
+        if (verified) verified[w] = v1 & valid;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// plain sieve kernel: one tile of tile_bits window bits per workgroup
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ __launch_bounds__(BS) void sieve_window_kernel(Tables T, uint64_t qv0, uint32_t lead, uint64_t nbits,
+                                                          uint32_t tile_bits, uint64_t* __restrict__ out,
+                                                          int tail_ones) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* win32 = lds;
+    uint32_t* med_i0 = lds + tile_bits / 32 + 2;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t g0 = (uint64_t)blockIdx.x * tile_bits;
+    if (g0 >= nbits) return;
+    const uint32_t tb = (uint32_t)(nbits - g0 < tile_bits ? nbits - g0 : tile_bits);
+    sieve_tile<BS>(win32, med_i0, qv0, lead, g0, tb, T, tid);
+    const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
+    const uint32_t nw = (tb + 63) >> 6;
+    const uint64_t wo = g0 >> 6;
+    for (uint32_t k = tid; k < nw; k += BS) {
+        uint64_t w = win64[k];
+        if (k == nw - 1 && (tb & 63)) {
+            const uint64_t tail = ~0ull << (tb & 63);
+            w = tail_ones ? (w | tail) : (w & ~tail);
+        }
+        out[wo + k] = w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// plain marking kernel: words_per_block verified words per workgroup; the
+// window slice (plus the 64-word halo) is staged HBM -> LDS once.
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ __launch_bounds__(BS) void mark_segment_kernel(Tables T, MarkParams mp, uint64_t n0, uint64_t cnt,
+                                                          const uint64_t* __restrict__ window, uint64_t window_words,
+                                                          uint32_t wpb, uint64_t* __restrict__ verified,
+                                                          uint32_t* __restrict__ pmin_out, SlowList sl,
+                                                          StatRec* __restrict__ rec) {
+    extern __shared__ uint32_t lds[];
+    uint64_t* win64 = reinterpret_cast<uint64_t*>(lds);
+    const uint32_t tid = threadIdx.x;
+    const uint64_t w0 = (uint64_t)blockIdx.x * wpb;          // first verified word
+    const uint64_t j0 = w0 * 64;
+    const uint32_t lcnt = (uint32_t)(cnt - j0 < (uint64_t)wpb * 64 ? cnt - j0 : (uint64_t)wpb * 64);
+    const uint32_t nwl = (lcnt + 63) >> 6;
+    const uint32_t stage = nwl + HALO / 64 + 1;
+    for (uint32_t k = tid; k < stage; k += BS) {
+        const uint64_t gw = w0 + k;
+        win64[k] = gw < window_words ? window[gw] : 0;
+    }
+    __syncthreads();
+    LaneStats st = lane_stats_zero();
+    mark_words<BS>(win64, n0 + 2 * j0, lcnt, mp, T.sp, verified ? verified + w0 : nullptr,
+                   pmin_out ? pmin_out + j0 : nullptr, sl, 0, st, tid);
+    block_store<BS>(st, lcnt, &rec[blockIdx.x], tid);
+}
+
+// ---------------------------------------------------------------------------
+// fused persistent kernel: tiles of tile_evens evens from a device counter;
+// sieve and marking share the LDS window, nothing but results leave the CU.
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, const WinDesc* __restrict__ wins,
+                                                   uint32_t nwin, uint64_t ntiles, uint32_t tile_evens,
+                                                   uint64_t* __restrict__ tile_counter,
+                                                   uint64_t* __restrict__ verified, SlowList sl,
+                                                   StatRec* __restrict__ rec) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t win_words32 = (tile_evens + HALO + WIN_PAD) / 32 + 2;
+    uint32_t* win32 = lds;
+    uint32_t* med_i0 = lds + win_words32;
+    __shared__ uint64_t s_tile;
+    const uint32_t tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd((unsigned long long*)tile_counter, 1ull);
+        __syncthreads();
+        const uint64_t t = s_tile;
+        __syncthreads();
+        if (t >= ntiles) break;
+        // window holding tile t
+        uint32_t lo = 0, hi = nwin;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (wins[mid].tile0 <= t) lo = mid; else hi = mid;
+        }
+        const WinDesc wd = wins[lo];
+        const uint64_t lt = t - wd.tile0;
+        const uint64_t n0 = wd.start + 2ull * tile_evens * lt;
+        const uint64_t rem = wd.evens - (uint64_t)tile_evens * lt;
+        const uint32_t cnt = (uint32_t)(rem < tile_evens ? rem : tile_evens);
+        const WinGeom g = window_geom(n0, (uint64_t)cnt + HALO + WIN_PAD);
+        sieve_tile<BS>(win32, med_i0, g.qv0, g.lead, 0, g.nbits, T, tid);
+        LaneStats st = lane_stats_zero();
+        mark_words<BS>(reinterpret_cast<const uint64_t*>(win32), n0, cnt, mp, T.sp,
+                       verified ? verified + wd.word0 + lt * (tile_evens / 64) : nullptr, nullptr, sl, lo, st,
+                       tid);
+        block_store<BS>(st, cnt, &rec[t], tid);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// slow path: entries whose p_min exceeds the window's reach (p > sp[K-1])
+// ---------------------------------------------------------------------------
+__global__ void slow_kernel(Tables T, MarkParams mp, SlowList sl, uint64_t* __restrict__ res_p,
+                            const WinDesc* __restrict__ wins, uint64_t* __restrict__ verified,
+                            uint32_t* __restrict__ pmin_out, uint64_t pmin_n0) {
+    const uint32_t cnt = *sl.count < sl.cap ? *sl.count : sl.cap;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint64_t n = sl.n[i];
+        const uint64_t half = n / 2;
+        uint64_t found = 0;
+        uint32_t k = mp.K;
+        for (; k < T.nsp; ++k) {
+            const uint64_t p = T.sp[k];
+            if (p > half || p > mp.bound) break;
+            if (is_prime_q(n - p, T)) { found = p; break; }
+        }
+        if (!found && k == T.nsp)
+            for (uint64_t p = (uint64_t)T.sp[T.nsp - 1] + 2; p <= half && p <= mp.bound; p += 2)
+                if (is_prime_u64(p) && is_prime_q(n - p, T)) { found = p; break; }
+        res_p[i] = found;
+        const WinDesc wd = wins[sl.win[i]];
+        const uint64_t j = (n - wd.start) / 2;
+        if (found && found <= mp.p_small && verified) {
+            const uint64_t bit = wd.word0 * 64 + j;
+            atomicOr((unsigned long long*)&verified[bit >> 6], 1ull << (bit & 63));
+        }
+        if (pmin_out) pmin_out[(n - pmin_n0) / 2] = (uint32_t)found;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reduce: one workgroup per window
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void reduce_kernel(MarkParams mp, const StatRec* __restrict__ rec,
+                                                      const uint64_t* __restrict__ rec_lo,
+                                                      const uint64_t* __restrict__ rec_hi, SlowList sl,
+                                                      const uint64_t* __restrict__ res_p, StatRec* __restrict__ out) {
+    const uint32_t w = blockIdx.x, tid = threadIdx.x;
+    LaneStats s = lane_stats_zero();
+    uint64_t evens = 0;
+    for (uint64_t r = rec_lo[w] + tid; r < rec_hi[w]; r += blockDim.x) {
+        const StatRec& x = rec[r];
+        s.ph2 += x.phase2;
+        s.fail += x.failures;
+        s.first = x.first_fail < s.first ? x.first_fail : s.first;
+        s.sum += x.sum_pmin;
+        merge_max(s.mx, s.arg, x.max_pmin, x.argmax);
+        evens += x.evens;
+    }
+    const uint32_t cnt = *sl.count < sl.cap ? *sl.count : sl.cap;
+    for (uint32_t i = tid; i < cnt; i += blockDim.x) {
+        if (sl.win[i] != w) continue;
+        const uint64_t n = sl.n[i], p = res_p[i];
+        if (p) {
+            if (p <= mp.p_small) s.ph2 -= 1;   // counted unverified by the window pass
+            if (mp.flags & 1u) {
+                s.sum += p;
+                merge_max(s.mx, s.arg, p, n);
+            }
+        } else {
+            s.fail += 1;
+            s.first = n < s.first ? n : s.first;
+        }
+    }
+    __shared__ uint64_t s_ev[256 / 64];
+    wave_reduce(s);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) evens += __shfl_xor(evens, o);
+    if ((tid & 63) == 0) s_ev[tid >> 6] = evens;
+    __shared__ LaneStats red[256 / 64];
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        LaneStats t = red[0];
+        uint64_t ev = s_ev[0];
+        for (int i = 1; i < 256 / 64; ++i) {
+            t.ph2 += red[i].ph2;
+            t.fail += red[i].fail;
+            t.first = red[i].first < t.first ? red[i].first : t.first;
+            t.sum += red[i].sum;
+            merge_max(t.mx, t.arg, red[i].mx, red[i].arg);
+            ev += s_ev[i];
+        }
+        StatRec r;
+        r.phase2 = t.ph2;
+        r.failures = t.fail;
+        r.first_fail = t.first;
+        r.sum_pmin = t.sum;
+        r.max_pmin = t.mx;
+        r.argmax = t.arg;
+        r.evens = ev;
+        r.pad = 0;
+        out[w] = r;
+    }
+}
+
+__global__ void fill_kernel(uint64_t* p, uint64_t words, uint64_t value) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// table builders
+// ---------------------------------------------------------------------------
+// bits_out[w] bit b <-> odd q = 2(64w+b)+3 <= limit, by trial division (tiny limits only)
+__global__ void seed_primes_kernel(uint32_t limit, uint64_t* bits_out) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nbits = (limit - 3) / 2 + 1;
+    if (w >= (nbits + 63) / 64) return;
+    uint64_t word = 0;
+    for (uint32_t b = 0; b < 64; ++b) {
+        const uint32_t i = 64 * w + b;
+        if (i >= nbits) break;
+        const uint32_t q = 2 * i + 3;
+        bool pr = true;
+        for (uint32_t d = 3; d * d <= q; d += 2)
+            if (q % d == 0) { pr = false; break; }
+        if (pr) word |= 1ull << b;
+    }
+    bits_out[w] = word;
+}
+
+// single workgroup: list of odd q <= qlimit whose bit is set (bit i <-> 2i+3)
+__global__ __launch_bounds__(1024) void compact_primes_kernel(const uint64_t* __restrict__ bits, uint32_t nwords,
+                                                              uint32_t qlimit, uint32_t* __restrict__ out,
+                                                              uint32_t* __restrict__ count_out) {
+    __shared__ uint32_t cnt[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (nwords + 1023) / 1024;
+    const uint32_t a = tid * per, b = a + per < nwords ? a + per : nwords;
+    const uint32_t nbits = (qlimit - 3) / 2 + 1;
+    uint32_t c = 0;
+    for (uint32_t w = a; w < b; ++w) {
+        uint64_t x = bits[w];
+        if (64ull * (w + 1) > nbits) x &= (64ull * w >= nbits) ? 0 : ((1ull << (nbits - 64 * w)) - 1);
+        c += __popcll(x);
+    }
+    cnt[tid] = c;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint32_t v = tid >= o ? cnt[tid - o] : 0;
+        __syncthreads();
+        cnt[tid] += v;
+        __syncthreads();
+    }
+    uint32_t pos = cnt[tid] - c;
+    for (uint32_t w = a; w < b; ++w) {
+        uint64_t x = bits[w];
+        if (64ull * (w + 1) > nbits) x &= (64ull * w >= nbits) ? 0 : ((1ull << (nbits - 64 * w)) - 1);
+        while (x) {
+            const uint32_t bb = __builtin_ctzll(x);
+            x &= x - 1;
+            out[pos++] = 2 * (64 * w + bb) + 3;
+        }
+    }
+    if (tid == 1023) *count_out = cnt[1023];
+}
+
+__global__ void prime_aux_kernel(const uint32_t* __restrict__ sp, uint32_t n, double* __restrict__ inv,
+                                 uint32_t* __restrict__ c32) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t p = sp[k];
+    inv[k] = 1.0 / (double)p;
+    c32[k] = (uint32_t)((1ull << 32) % p);
+}
+
+// ---------------------------------------------------------------------------
+// serving: one wave per query; lanes try 64 consecutive primes, base-2 SPRP
+// filters, the lowest surviving lane is confirmed with the 12 bases spread
+// over 12 lanes.  The answer is the minimal p (scan order of cpu_goldbach.cpp:19-22).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool small_factor(uint64_t q) {
+    return q % 3 == 0 || q % 5 == 0 || q % 7 == 0 || q % 11 == 0 || q % 13 == 0 || q % 17 == 0 ||
+           q % 19 == 0 || q % 23 == 0;
+}
+
+__global__ __launch_bounds__(64 * SERVE_WAVES) void min_partition_kernel(Tables T, const uint64_t* __restrict__ ns,
+                                                                         uint32_t count, uint64_t bound,
+                                                                         uint64_t* __restrict__ p_out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t qi = blockIdx.x * SERVE_WAVES + (threadIdx.x >> 6);
+    if (qi >= count) return;   // uniform per wave
+    const uint64_t n = ns[qi];
+    uint64_t res = 0;
+    if (n >= 4 && !(n & 1)) {
+        if (n == 4) {
+            res = bound >= 2 ? 2 : 0;
+        } else {
+            const uint64_t half = n / 2;
+            bool done = false;
+            for (uint64_t base = 0; !done; base += 64) {
+                const uint64_t k = base + lane;
+                uint64_t p;
+                bool isp;
+                if (k < T.nsp) {
+                    p = T.sp[k];
+                    isp = true;
+                } else {
+                    p = (uint64_t)T.sp[T.nsp - 1] + 2 * (k - T.nsp + 1);
+                    isp = p <= half && p <= bound && is_prime_u64(p);
+                }
+                const bool in = p <= half && p <= bound;
+                if (__ballot(in) == 0) break;
+                const uint64_t q = n - p;
+                bool pass = false;
+                if (in && isp) {
+                    if (q <= T.p2_limit) {
+                        pass = is_prime_q(q, T);
+                    } else if (!small_factor(q)) {
+                        const Mont M = mont_setup(q);
+                        const uint32_t r = __builtin_ctzll(q - 1);
+                        pass = sprp(2, (q - 1) >> r, r, M);
+                    }
+                }
+                uint64_t ball = __ballot(pass);
+                while (ball) {
+                    const uint32_t L = __ffsll((unsigned long long)ball) - 1;
+                    const uint64_t qq = __shfl(q, L);
+                    bool ok = true;
+                    if (qq > T.p2_limit && lane < 12) {
+                        const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+                        const Mont M = mont_setup(qq);
+                        const uint32_t r = __builtin_ctzll(qq - 1);
+                        ok = sprp(bases[lane], (qq - 1) >> r, r, M);
+                    }
+                    if (__all(ok)) {
+                        res = __shfl(p, L);
+                        done = true;
+                        break;
+                    }
+                    ball &= ball - 1;
+                }
+            }
+        }
+    }
+    if (lane == 0) p_out[qi] = res;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename K>
+static hipError_t allow_lds(K kernel, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
+hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uint32_t lead, uint64_t nbits,
+                               uint32_t tile_bits, uint64_t* out, int tail_ones) {
+    if (nbits == 0) return hipSuccess;
+    const uint64_t ntiles = (nbits + tile_bits - 1) / tile_bits;
+    if (ntiles > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(tile_bits / 32 + 2) * 4 + (size_t)(T.n_med > NPAT ? T.n_med - NPAT : 1) * 4;
+    hipError_t e = allow_lds(sieve_window_kernel<SIEVE_BS>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sieve_window_kernel<SIEVE_BS>, dim3((uint32_t)ntiles), dim3(SIEVE_BS), lds, s, T, qv0, lead,
+                       nbits, tile_bits, out, tail_ones);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams& mp, uint64_t n0, uint64_t cnt,
+                               const uint64_t* window, uint64_t window_words, uint32_t wpb, uint64_t* verified,
+                               uint32_t* pmin_out, SlowList sl, StatRec* rec, uint32_t* nrec_out) {
+    const uint64_t nw = (cnt + 63) / 64;
+    const uint64_t blocks = (nw + wpb - 1) / wpb;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(wpb + HALO / 64 + 1) * 8;
+    hipError_t e = allow_lds(mark_segment_kernel<MARK_BS>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mark_segment_kernel<MARK_BS>, dim3((uint32_t)blocks), dim3(MARK_BS), lds, s, T, mp, n0, cnt,
+                       window, window_words, wpb, verified, pmin_out, sl, rec);
+    if (nrec_out) *nrec_out = (uint32_t)blocks;
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
+                        uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
+                        uint64_t* verified, SlowList sl, StatRec* rec) {
+    const size_t lds = (size_t)((tile_evens + HALO + WIN_PAD) / 32 + 2) * 4 +
+                       (size_t)(T.n_med > NPAT ? T.n_med - NPAT : 1) * 4;
+    hipError_t e = allow_lds(fused_kernel<FUSED_BS>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fused_kernel<FUSED_BS>, dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin, ntiles,
+                       tile_evens, tile_counter, verified, sl, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
+                       const WinDesc* wins, uint64_t* verified, uint32_t* pmin_out, uint64_t pmin_n0) {
+    hipLaunchKernelGGL(slow_kernel, dim3(256), dim3(256), 0, s, T, mp, sl, res_p, wins, verified, pmin_out, pmin_n0);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(hipStream_t s, const MarkParams& mp, const StatRec* rec, const uint64_t* rec_lo,
+                         const uint64_t* rec_hi, uint32_t nwin, SlowList sl, const uint64_t* res_p, StatRec* out) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(nwin), dim3(256), 0, s, mp, rec, rec_lo, rec_hi, sl, res_p, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t s, uint64_t* p, uint64_t words, uint64_t value) {
+    if (!words) return hipSuccess;
+    uint64_t blocks = (words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, words, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_primes(hipStream_t s, uint32_t limit, uint64_t* bits_out) {
+    const uint32_t nbits = (limit - 3) / 2 + 1;
+    const uint32_t nw = (nbits + 63) / 64;
+    hipLaunchKernelGGL(seed_primes_kernel, dim3((nw + 63) / 64), dim3(64), 0, s, limit, bits_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t nwords, uint32_t qlimit,
+                                 uint32_t* primes_out, uint32_t* count_out) {
+    hipLaunchKernelGGL(compact_primes_kernel, dim3(1), dim3(1024), 0, s, bits, nwords, qlimit, primes_out, count_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32) {
+    hipLaunchKernelGGL(prime_aux_kernel, dim3((n + 255) / 256), dim3(256), 0, s, sp, n, inv, c32);
+    return hipGetLastError();
+}
+
+hipError_t launch_min_partition(hipStream_t s, const Tables& T, const uint64_t* ns, uint32_t count, uint64_t bound,
+                                uint64_t* p_out) {
+    if (!count) return hipSuccess;
+    const uint32_t blocks = (count + SERVE_WAVES - 1) / SERVE_WAVES;
+    hipLaunchKernelGGL(min_partition_kernel, dim3(blocks), dim3(64 * SERVE_WAVES), 0, s, T, ns, count, bound, p_out);
+    return hipGetLastError();
+}
+
+}  // namespace gbk

@@ -1,0 +1,82 @@
+// gb_kernels.hpp -- host-side view of the HIP kernels in gb_kernels.hip.
+// Launch wrappers only; no device code here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gb_device.hpp"
+
+namespace gbk {
+
+constexpr int SIEVE_BS = 512;    // threads per plain-sieve workgroup
+constexpr int MARK_BS = 512;     // threads per plain-marking workgroup
+constexpr int FUSED_BS = 1024;   // threads per fused persistent workgroup
+constexpr int SERVE_WAVES = 4;   // queries per serving workgroup (one wave each)
+
+struct MarkParams {
+    uint64_t p_small;
+    uint64_t bound;       // max(p_small, phase2_limit)
+    uint32_t K;           // odd marking primes sp[0..K), all <= min(bound, HALO_P)
+    uint32_t fast_final;  // 1 when sp[0..K) reaches bound: unresolved n are failures
+    uint32_t flags;       // bit 0: p_min statistics
+    uint32_t pad;
+};
+
+struct SlowList {
+    uint64_t* n;          // unresolved even n (p_min above the window's reach)
+    uint32_t* win;        // window index of each entry
+    uint32_t* count;      // entries appended (may exceed cap: then *overflow is set)
+    uint32_t* overflow;
+    uint32_t cap;
+    uint32_t pad;
+};
+
+// One work window for the fused kernel: `evens` even n from `start`.
+struct WinDesc {
+    uint64_t start;
+    uint64_t evens;
+    uint64_t tile0;       // global index of the window's first tile
+    uint64_t word0;       // first word of the window in the optional verified map
+};
+
+// ---- launchers (stream-ordered, no synchronisation) ----
+
+// Odd-only sieve of window bits [0, nbits): bit i <-> q = qv0 + 2(i - lead), i >= lead.
+hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uint32_t lead, uint64_t nbits,
+                               uint32_t tile_bits, uint64_t* out, int tail_ones);
+
+// Plain marking of one segment from a window already in device memory.
+hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams& mp, uint64_t n0, uint64_t cnt,
+                               const uint64_t* window, uint64_t window_words, uint32_t words_per_block,
+                               uint64_t* verified, uint32_t* pmin_out, SlowList sl, StatRec* rec,
+                               uint32_t* nrec_out);
+
+// Fused persistent kernel over the tiles of `nwin` windows.
+hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
+                        uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
+                        uint64_t* verified, SlowList sl, StatRec* rec);
+
+// Resolve slow-list entries: search primes above sp[K-1] up to bound.
+hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
+                       const WinDesc* wins, uint64_t* verified, uint32_t* pmin_out, uint64_t pmin_n0);
+
+// Merge per-block records of window w in [rec_lo[w], rec_hi[w]) plus its slow entries.
+hipError_t launch_reduce(hipStream_t s, const MarkParams& mp, const StatRec* rec, const uint64_t* rec_lo,
+                         const uint64_t* rec_hi, uint32_t nwin, SlowList sl, const uint64_t* res_p,
+                         StatRec* out);
+
+// Zero a set of 64-bit words (graph-safe replacement for hipMemsetAsync).
+hipError_t launch_fill(hipStream_t s, uint64_t* p, uint64_t words, uint64_t value);
+
+// Table builders.
+hipError_t launch_seed_primes(hipStream_t s, uint32_t limit, uint64_t* bits_out);
+hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t nwords, uint32_t qlimit,
+                                 uint32_t* primes_out, uint32_t* count_out);
+hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32);
+
+// Serving: p_out[i] = minimal p for ns[i].
+hipError_t launch_min_partition(hipStream_t s, const Tables& T, const uint64_t* ns, uint32_t count, uint64_t bound,
+                                uint64_t* p_out);
+
+}  // namespace gbk
