@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark: even n verified per second over [4, 1e12] on one MI355X.
+
+One step = one complete verification pass over all 499,999,999,999 even n in
+[4, 1e12] with P_SMALL = 1e6, in the batched persistent form (one fused
+sieve+mark kernel over 2^20-even tiles from a device queue, replayed as a
+hipGraph).  The plain per-segment form (2^27 evens per segment, one host sync
+per segment, as the reference's loop) is timed on the same range beside it.
+
+The JSON line carries:
+  roofline      the fused kernel: algorithmic bytes (0.375 B per even n:
+                window bitset written + read, verified bitmap written) over
+                its average launch time, measured with HIP events on the
+                library's stream around each launch; traffic from the
+                committed rocprofv3 counter summary when present.
+  cpu_baseline  the CPU restatement (oracle/, OpenMP over segments, the
+                reference algorithm: per-n ascending prime scan) timed on this
+                host over 32 sampled 2^24-even segments spread over [4, 1e12].
+  serving       single-query latency percentiles and batched queries/s of the
+                minimal-partition kernel on n log-uniform in [1e6, 2^64-2].
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+LIMIT = 10**12
+P_SMALL = 10**6
+PLAIN_SEG = 1 << 27
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES_PER_EVEN = 0.375
+REF_PUBLISHED_EVENS_PER_S = 499_999_999_999 / 37.440   # RESULTS.md:48, 1x RTX 5090
+METRIC = "even n verified/s at N=1e12 on 1 GPU, batched vs plain; p99 query latency"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_kernel_1e12_traffic.json")
+
+
+def device_sync() -> None:
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def timed_passes(fn, steps: int, warmup: int):
+    for _ in range(warmup):
+        fn()
+    device_sync()
+    t0 = time.perf_counter()
+    outs = [fn() for _ in range(steps)]
+    device_sync()
+    return time.perf_counter() - t0, outs
+
+
+def cpu_baseline() -> dict:
+    from _helpers import Oracle
+
+    orc = Oracle()
+    threads = min(16, os.cpu_count() or 1)
+    seg = 1 << 24
+    starts = [4 + 2 * ((k * (LIMIT // 2 - seg)) // 31) for k in range(32)]
+    evens = 0
+    t0 = time.perf_counter()
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(a):
+        st, _, _ = orc.verify_segment(a, seg, P_SMALL, want_bits=False, want_pmin=False)
+        return st["evens"], st["failures"]
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        for e, f in ex.map(one, starts):
+            evens += e
+            assert f == 0
+    dt = time.perf_counter() - t0
+    return {"value": evens / dt, "unit": "evens/s", "cores": threads, "kind": "port",
+            "sample": f"32 segments of 2^24 evens spread over [4, 1e12], {threads} threads, {dt:.1f} s wall"}
+
+
+def serving(ctx) -> dict:
+    rng = np.random.default_rng(1)
+    n = (np.exp(rng.uniform(np.log(1e6), np.log(2.0**64 - 2), size=65536)) / 2).astype(np.uint64) * np.uint64(2)
+    ctx.min_partition(n[:64])   # warm
+    lat = []
+    for i in range(400):
+        t0 = time.perf_counter()
+        ctx.min_partition(n[i:i + 1])
+        lat.append((time.perf_counter() - t0) * 1e3)
+    t0 = time.perf_counter()
+    ctx.min_partition(n)
+    dt = time.perf_counter() - t0
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "batch_qps": 65536 / dt, "batch": 65536, "mode": "single-query latency; 65536-query batch throughput"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build verifies on one GPU; run with --gpus 1 (multi-device sharding "
+                                   "is a design note, not built)"}))
+        return 2
+
+    from _helpers import load_pkg
+
+    gb = load_pkg()
+    ctx = gb.Context(0, max_seg_evens=PLAIN_SEG)
+    evens = (LIMIT - 4) // 2 + 1
+
+    def batched():
+        st = ctx.verify_range(4, LIMIT, P_SMALL, form=gb.GB_FORM_BATCHED)
+        assert st["evens"] == evens and st["failures"] == 0 and st["phase2"] == 0
+        return st
+
+    dt, outs = timed_passes(batched, args.steps, args.warmup)
+    value = evens * args.steps / dt
+    kms = float(np.mean([o["kernel_ms"] for o in outs]))
+    achieved = ALG_BYTES_PER_EVEN * evens / (kms * 1e-3) / 1e9
+
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    plain = None
+    if not args.no_plain:
+        def plain_pass():
+            st = ctx.verify_range(4, LIMIT, P_SMALL, form=gb.GB_FORM_PLAIN, seg_evens=PLAIN_SEG)
+            assert st["evens"] == evens and st["failures"] == 0
+            return st
+        psteps = max(1, min(args.steps, 2))
+        pdt, pouts = timed_passes(plain_pass, psteps, 0)
+        plain = {"value": evens * psteps / pdt, "ms_per_step": pdt * 1e3 / psteps,
+                 "kernel_ms": float(np.mean([o["kernel_ms"] for o in pouts])), "seg_evens": PLAIN_SEG}
+
+    serve = serving(ctx)
+    ctx.close()
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evens/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / REF_PUBLISHED_EVENS_PER_S,
+        "dtype": "u64",
+        "data": "synthetic (the input is the integer range itself)",
+        "config": {"workload": "all even n in [4, 1e12], batched persistent form", "limit": LIMIT,
+                   "p_small": P_SMALL, "tile_evens": 1 << 20, "parallelism": "1 GPU",
+                   "plain_seg_evens": PLAIN_SEG},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "fused_kernel", "kernel_ms": kms,
+                     "alg_bytes_per_launch": ALG_BYTES_PER_EVEN * evens},
+        "cpu_baseline": cpu,
+        "plain": plain,
+        "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "serving": serve,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,49 @@
+"""Device sieve parity: gb_sieve_window against the oracle's window bits
+(reference kernel contract: src/goldbach.cu:198-259, tail bits 1) and the
+PrimeBitset layout (include/prime_bitset.hpp of the reference)."""
+import numpy as np
+import pytest
+
+from _helpers import unpack
+
+pytestmark = pytest.mark.gpu
+
+WINDOWS = [
+    (3, 3), (3, 5), (3, 127), (3, 129), (3, 131), (3, 1001), (61, 63), (67, 4491), (3, 10**6 + 1),
+    (4489, 4489 + 2 * 64 * 1000 - 2),               # starts at 67^2
+    (10**9 + 1, 10**9 + 200_001),
+    (999_999_999_001, 1_000_000_077_777),           # around 1e12 (sieve with p <= 1e6)
+    (17_592_186_044_417, 17_592_186_244_417),       # just above 2^44: pre-sieve + Miller-Rabin
+    (17_592_177_655_809, 17_592_186_244_417),       # straddles 2^44: some tiles sieve, some test
+    (10**18 + 1, 10**18 + 100_001),
+    (2**64 - 2**21 + 1, 2**64 - 1),                 # top of the 64-bit range
+]
+
+
+@pytest.mark.parametrize("q_low,q_high", WINDOWS)
+def test_window_matches_oracle(ctx, orc, q_low, q_high):
+    got = ctx.sieve_window(q_low, q_high)
+    want = orc.window_bits(q_low, q_high)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("limit", [3, 4, 5, 9, 30, 127, 128, 129, 130, 10**6, 10**6 + 1, 10**8])
+def test_prime_bitset_layout(ctx, orc, limit):
+    q_high = limit if limit & 1 else limit - 1
+    got = ctx.sieve_window(3, q_high)
+    assert np.array_equal(got, orc.prime_bitset(limit))
+
+
+def test_pi_1e9_full_window(ctx):
+    # chunked through the segment buffer; pi(1e9) = 50,847,534 (2 is not in the odd set)
+    bits = ctx.sieve_window(3, 999_999_999)
+    nbits = (999_999_999 - 3) // 2 + 1
+    assert int(unpack(bits, nbits).sum()) == 50_847_534 - 1
+
+
+@pytest.mark.parametrize("q_low,q_high", [(2, 11), (1, 11), (5, 4), (3, 10), (4, 11)])
+def test_window_rejects_bad_bounds(ctx, gb, q_low, q_high):
+    with pytest.raises(gb.GoldbachError) as e:
+        ctx.sieve_window(q_low, q_high)
+    assert e.value.status == gb.GB_EINVAL

@@ -1,0 +1,143 @@
+"""Device verifier parity against the oracle: per-segment verified bitmaps,
+per-n minimal partitions, Phase-2 and failure counts (reference semantics:
+src/goldbach.cu:288-311 Phase 1, :362-373 Phase 2, :420-485 segment loop),
+the batched persistent form against the plain form, windows that need
+Miller-Rabin, and single-number minimal partitions (single_check's job)."""
+import numpy as np
+import pytest
+
+from _helpers import unpack
+
+pytestmark = pytest.mark.gpu
+
+SEGMENTS = [
+    (4, 1, 10**6), (4, 2, 10**6), (6, 1, 10**6), (4, 63, 10**6), (4, 64, 10**6), (4, 65, 10**6),
+    (4, 1000, 0), (4, 1000, 2), (4, 1000, 3), (4, 1000, 5), (4, 1000, 7), (4, 5000, 100),
+    (4, 200_000, 10**6), (8190, 5000, 10**6), (8194, 5000, 10**6), (10**6, 100_000, 1000),
+    (10**9, 1 << 17, 10**6), (10**9 + 2, 12_345, 8193), (10**9 + 2, 12_345, 8194),
+    (10**12 - 2 * (1 << 20), 1 << 20, 10**6), (10**12, 3000, 4 * 10**9),
+    (10**15, 20_000, 10**6), (10**18 + 2, 10_000, 10**6),
+    (2**64 - 2 - 2 * 4095, 4096, 10**6),
+]
+
+
+@pytest.mark.parametrize("A,count,p_small", SEGMENTS)
+def test_segment_matches_oracle(ctx, orc, A, count, p_small):
+    r = ctx.verify_segment(A, count, p_small, want_pmin_array=True)
+    st, bits, pmin = orc.verify_segment(A, count, p_small)
+    assert np.array_equal(r.verified, bits)
+    assert np.array_equal(r.pmin, pmin)
+    for k in ("evens", "phase2", "failures", "first_fail_n", "sum_pmin", "max_pmin", "argmax_n"):
+        assert r.stats[k] == st[k], k
+
+
+def test_segment_slow_path_record(ctx, orc):
+    # p_min(3325581707333960528) = 9781 lies beyond the 8193 the window serves
+    n = 3325581707333960528
+    A = n - 2 * 300
+    r = ctx.verify_segment(A, 700, 10**6, want_pmin_array=True)
+    st, bits, pmin = orc.verify_segment(A, 700, 10**6)
+    assert int(r.pmin[300]) == 9781
+    assert np.array_equal(r.pmin, pmin) and np.array_equal(r.verified, bits)
+    assert (r.stats["max_pmin"], r.stats["argmax_n"]) == (9781, n)
+    # with p_small below the record the slow path reports Phase 2 for it
+    r2 = ctx.verify_segment(A, 700, 9000, want_pmin_array=True)
+    st2, bits2, _ = orc.verify_segment(A, 700, 9000)
+    assert np.array_equal(r2.verified, bits2) and r2.stats["phase2"] == st2["phase2"] == 1
+
+
+def test_segment_bad_arguments(ctx, gb):
+    for args in [(5, 10, 100), (2, 10, 100), (4, 0, 100), (2**64 - 2, 2, 100)]:
+        with pytest.raises(gb.GoldbachError) as e:
+            ctx.verify_segment(*args)
+        assert e.value.status == gb.GB_EINVAL
+    with pytest.raises(gb.GoldbachError) as e:
+        ctx.verify_segment(4, (1 << 27) + 1, 100)
+    assert e.value.status == gb.GB_ERANGE
+
+
+def test_failure_hook(gb, orc):
+    # Phase 2 cut to p <= 7: the first failing n is the record n = 98
+    with gb.Context(0, phase2_limit=7) as c:
+        r = c.verify_segment(4, 499, 3, want_pmin_array=True)
+        st, bits, pmin = orc.verify_segment(4, 499, 3, phase2_limit=7)
+        assert r.stats["first_fail_n"] == 98
+        for k in ("phase2", "failures", "first_fail_n", "sum_pmin"):
+            assert r.stats[k] == st[k]
+        assert np.array_equal(r.pmin, pmin) and np.array_equal(r.verified, bits)
+        for form in (gb.GB_FORM_PLAIN, gb.GB_FORM_BATCHED):
+            rr = c.verify_range(4, 1000, 3, form=form)
+            assert rr["first_fail_n"] == 98 and rr["failures"] > 0
+
+
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_range_1e8_known_sums(ctx, gb, form):
+    f = gb.GB_FORM_PLAIN if form == "plain" else gb.GB_FORM_BATCHED
+    st = ctx.verify_range(4, 10**8, 10**6, form=f, want_pmin=True)
+    assert st["evens"] == 49_999_999 and st["phase2"] == 0 and st["failures"] == 0
+    assert st["sum_pmin"] == 1_511_603_116
+    assert (st["max_pmin"], st["argmax_n"]) == (1093, 60119912)
+
+
+@pytest.mark.parametrize("tile_log2", [12, 14, 16, 18, 19, 20])
+def test_batched_tile_sizes_agree(ctx, gb, tile_log2):
+    st = ctx.verify_range(10**6, 10**8 + 10**6, 1000, form=gb.GB_FORM_BATCHED, tile_log2=tile_log2,
+                          want_pmin=True)
+    ref = ctx.verify_range(10**6, 10**8 + 10**6, 1000, form=gb.GB_FORM_PLAIN, seg_evens=10**7, want_pmin=True)
+    for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
+        assert st[k] == ref[k], k
+
+
+def test_range_matches_oracle_with_start(ctx, gb, orc):
+    # validation_gpu.sh 1.1/1.3 settings: --seg-size=1e6 --p-small=1e5, start 1e6
+    ref = orc.verify_range(1_000_000, 10**9, 10**6, 100_000)
+    for form in (gb.GB_FORM_PLAIN, gb.GB_FORM_BATCHED):
+        st = ctx.verify_range(1_000_000, 10**9, 100_000, form=form, seg_evens=10**6, want_pmin=True)
+        for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
+            assert st[k] == ref[k], (form, k)
+
+
+def test_range_small_p_small_counts_phase2(ctx, gb, orc):
+    ref = orc.verify_range(4, 2 * 10**7, 1 << 22, 31)
+    for form in (gb.GB_FORM_PLAIN, gb.GB_FORM_BATCHED):
+        st = ctx.verify_range(4, 2 * 10**7, 31, form=form, seg_evens=1 << 22, want_pmin=True)
+        for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
+            assert st[k] == ref[k], (form, k)
+    assert ref["phase2"] > 0
+
+
+def test_full_size_1e11_plain_equals_batched(ctx, gb):
+    # the plain per-segment form at 2^27 evens per segment and the batched form
+    # share no kernel launch geometry; their per-range sums must agree exactly
+    b = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED, want_pmin=True)
+    p = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_PLAIN, seg_evens=1 << 27, want_pmin=True)
+    assert b["evens"] == 49_999_999_999 and b["failures"] == 0 and b["phase2"] == 0
+    for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
+        assert b[k] == p[k], k
+
+
+def test_windows_match_oracle(ctx, orc):
+    wins = [(10**15, 20_000), (10**18 + 2, 5_000), (1234567890123456, 7_777), (2**64 - 2 - 2 * 3999, 4000),
+            (4, 3000), (3325581707333960528 - 600, 700)]
+    per, tot = ctx.verify_windows(wins, 10**6, want_pmin=True)
+    evens = 0
+    for (a, e), got in zip(wins, per):
+        st, _, _ = orc.verify_segment(a, e, 10**6, want_bits=False, want_pmin=False)
+        for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
+            assert got[k] == st[k], (a, k)
+        evens += e
+    assert tot["evens"] == evens
+
+
+def test_min_partition_matches_oracle(ctx, orc):
+    rng = np.random.default_rng(1)
+    n = (np.exp(rng.uniform(np.log(1e6), np.log(2.0**64 - 2), size=3000)) / 2).astype(np.uint64) * np.uint64(2)
+    n = np.concatenate([n, np.array([4, 6, 8, 10, 98, 2**64 - 2, 3325581707333960528, 10**19], dtype=np.uint64)])
+    got = ctx.min_partition(n)
+    want = orc.min_partition_batch(n)
+    assert np.array_equal(got, want)
+    # odd or too small: no partition reported
+    assert list(ctx.min_partition([2, 3, 7, 0])) == [0, 0, 0, 0]
+    # each result is the same alone and in the mix
+    for i in (0, 17, 2999):
+        assert ctx.min_partition(n[i:i + 1])[0] == got[i]
