@@ -20,6 +20,8 @@
 // (segment sieve), :288-311 (Phase 1), :362-373 (Phase 2), :184-195 (count).
 #include "gb_kernels.hpp"
 
+#include <utility>
+
 namespace gbk {
 
 // ---------------------------------------------------------------------------
@@ -228,8 +230,79 @@ __device__ void block_store(LaneStats s, uint64_t evens, StatRec* out, uint32_t 
 }
 
 // ---------------------------------------------------------------------------
+// Unrolled marking over the odd primes p <= HALO_P.  Every such p reads the
+// window at the fixed offset HALO - (p-1)/2, so word index and shift are
+// compile-time constants: LDS reads carry immediate offsets (repeated words
+// are loaded once) and each prime costs two alignbit + two OR per lane.
+// ---------------------------------------------------------------------------
+constexpr int NMARK = 1027;   // odd primes in [3, 8193]
+
+struct MarkTable {
+    uint16_t p[NMARK];
+};
+
+constexpr MarkTable make_mark_table() {
+    MarkTable t{};
+    int n = 0;
+    for (uint32_t q = 3; q <= HALO_P && n < NMARK; q += 2) {
+        bool pr = true;
+        for (uint32_t d = 3; d * d <= q; d += 2)
+            if (q % d == 0) { pr = false; break; }
+        if (pr) t.p[n++] = (uint16_t)q;
+    }
+    return t;
+}
+
+constexpr MarkTable kMark = make_mark_table();
+static_assert(kMark.p[0] == 3 && kMark.p[NMARK - 1] == 8191, "odd primes <= 8193");
+
+template <uint32_t S>
+__device__ __forceinline__ uint64_t funnel_c(uint64_t lo, uint64_t hi) {
+    if constexpr (S == 0) return lo;
+    else return (lo >> S) | (hi << (64 - S));
+}
+
+template <int K, bool PM>
+__device__ __forceinline__ void mark_one(const uint64_t* __restrict__ w, uint64_t& v, LaneStats& st, uint64_t nbase) {
+    if constexpr (K < NMARK) {
+        constexpr uint32_t p = kMark.p[K];
+        constexpr uint32_t off = HALO - (p - 1) / 2;
+        const uint64_t R = funnel_c<off & 63>(w[off >> 6], w[(off >> 6) + 1]);
+        if constexpr (PM) {
+            const uint64_t nb = R & ~v;
+            if (nb) {
+                st.sum += (uint64_t)__popcll(nb) * p;
+                merge_max(st.mx, st.arg, p, nbase + 2ull * __builtin_ctzll(nb));
+            }
+        }
+        v |= R;
+    }
+}
+
+template <int K, bool PM, int... I>
+__device__ __forceinline__ void mark_group(const uint64_t* __restrict__ w, uint64_t& v, LaneStats& st,
+                                           uint64_t nbase, std::integer_sequence<int, I...>) {
+    (mark_one<K + I, PM>(w, v, st, nbase), ...);
+}
+
+// primes K, K+1, ... in groups of 8 until the wave's 64 words are all set
+template <int K, bool PM>
+__device__ __forceinline__ void mark_run(const uint64_t* __restrict__ w, uint64_t& v, LaneStats& st, uint64_t nbase) {
+    mark_group<K, PM>(w, v, st, nbase, std::make_integer_sequence<int, 8>{});
+    if constexpr (K + 8 < NMARK) {
+        // keeps each group's window reads inside the group (no hoisting of
+        // all ~65 words into registers up front)
+        asm volatile("" ::: "memory");
+        if (!__all(v == ~0ull)) mark_run<K + 8, PM>(w, v, st, nbase);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // mark_words: evens n = n_first + 2j, j in [0, cnt).  win64 bit (HALO + j - (p-1)/2)
 // holds q = n - p.  Lanes own 64-even words; the prime loop is wave-uniform.
+// The unrolled path serves p_small >= HALO_P (every marking prime is a
+// Phase-1 prime); the table-driven loop serves the rest and the per-n p_min
+// output.
 // ---------------------------------------------------------------------------
 template <int BS>
 __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first, uint32_t cnt,
@@ -239,6 +312,7 @@ __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first,
     const uint32_t lane = tid & 63, wave = tid >> 6;
     const uint32_t nw = (cnt + 63) >> 6;
     const bool want = mp.flags & 1u;
+    const bool fast = mp.K == NMARK && mp.p_small >= HALO_P && pmin_out == nullptr && !want;
     for (uint32_t base = wave * 64; base < nw; base += BS) {
         const uint32_t w = base + lane;
         const bool act = w < nw;
@@ -257,6 +331,12 @@ __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first,
         }
         uint64_t v1 = ~valid | ((has4 && mp.p_small >= 2) ? 1ull : 0ull);
         bool crossed = mp.p_small < 3;
+        if (fast) {
+            const uint64_t nbase = n_first + 128ull * w;
+            mark_run<0, false>(win64 + wr, v, st, nbase);
+            v1 = v;
+            crossed = true;
+        } else
         for (uint32_t k = 0; k < mp.K; ++k) {
             const uint32_t p = sp[k];
             if (!crossed && p > mp.p_small) {

@@ -29,6 +29,11 @@ def test_segment_matches_oracle(ctx, orc, A, count, p_small):
     assert np.array_equal(r.pmin, pmin)
     for k in ("evens", "phase2", "failures", "first_fail_n", "sum_pmin", "max_pmin", "argmax_n"):
         assert r.stats[k] == st[k], k
+    # without per-n outputs the unrolled marking path runs (p_small >= 8193)
+    f = ctx.verify_segment(A, count, p_small)
+    assert np.array_equal(f.verified, bits)
+    for k in ("evens", "phase2", "failures", "first_fail_n"):
+        assert f.stats[k] == st[k], k
 
 
 def test_segment_slow_path_record(ctx, orc):
@@ -114,6 +119,14 @@ def test_full_size_1e11_plain_equals_batched(ctx, gb):
     assert b["evens"] == 49_999_999_999 and b["failures"] == 0 and b["phase2"] == 0
     for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
         assert b[k] == p[k], k
+    # the same range through the unrolled marking path (no p_min statistics)
+    for form in (gb.GB_FORM_BATCHED, gb.GB_FORM_PLAIN):
+        f = ctx.verify_range(4, 10**11, 10**6, form=form, seg_evens=1 << 27)
+        assert (f["evens"], f["phase2"], f["failures"]) == (49_999_999_999, 0, 0)
+    # a small P_SMALL forces Phase 2 counts that both paths must agree on
+    q = ctx.verify_range(4, 10**9, 100, form=gb.GB_FORM_BATCHED)
+    r = ctx.verify_range(4, 10**9, 100, form=gb.GB_FORM_BATCHED, want_pmin=True)
+    assert q["phase2"] == r["phase2"] > 0
 
 
 def test_windows_match_oracle(ctx, orc):
