@@ -31,6 +31,7 @@ constexpr uint32_t MARK_WPB = 2048;                      // plain marking: verif
 constexpr uint32_t DEFAULT_TILE_LOG2 = 20;               // fused tile: 2^20 evens
 constexpr uint32_t SLOW_CAP = 1u << 20;
 constexpr uint32_t SEED_LIMIT = 4095;                    // trial-division seed (covers sqrt(2^22))
+constexpr uint32_t SP_TABLE_CAP = 300000;                // odd primes below 2^22: 295,946
 
 std::mutex g_open_mu;
 std::string g_open_err;
@@ -39,6 +40,7 @@ struct ItemTable {
     uint32_t tb = 0;        // tile bits the schedule was cut for
     uint32_t n_med = 0;
     uint32_t* d_items = nullptr;
+    uint32_t* d_items_p = nullptr;
     uint32_t* d_start = nullptr;
 };
 
@@ -82,6 +84,8 @@ struct gb_ctx {
     uint64_t* d_rec_hi = nullptr;
     uint32_t wins_cap = 0;
     uint64_t* d_counter = nullptr;
+    uint32_t* d_r0 = nullptr;          // per-window residues rbase mod sp[k]
+    uint64_t r0_cap = 0;
     // serving buffers
     uint64_t* d_qn = nullptr;
     uint64_t* d_qp = nullptr;
@@ -136,14 +140,17 @@ int build_items(gb_ctx* c, uint32_t tb, ItemTable* out) {
         if (it.tb == tb) { *out = it; return GB_OK; }
     ItemTable t;
     t.tb = tb;
-    std::vector<uint32_t> items, start;
+    std::vector<uint32_t> items, items_p, start;
     uint32_t k = NPAT;
     start.assign(NPAT + 1, 0);
     while (k < c->nsp && (uint64_t)c->h_sp[k] * CHUNK < tb) {
         const uint32_t p = c->h_sp[k];
         const uint32_t chunks = (uint32_t)((tb + (uint64_t)CHUNK * p - 1) / ((uint64_t)CHUNK * p));
         if (chunks > 4096) return fail(c, GB_EINVAL, "tile too large for the medium-prime schedule");
-        for (uint32_t ch = 0; ch < chunks; ++ch) items.push_back((k << 12) | ch);
+        for (uint32_t ch = 0; ch < chunks; ++ch) {
+            items.push_back((k << 12) | ch);
+            items_p.push_back(p);
+        }
         ++k;
         start.push_back((uint32_t)items.size());
     }
@@ -151,9 +158,12 @@ int build_items(gb_ctx* c, uint32_t tb, ItemTable* out) {
     // start[k'] for k' in [0, n_med]: items of primes with index < k'
     start.resize(t.n_med + 1);
     HCK(c, dalloc(&t.d_items, items.size()));
+    HCK(c, dalloc(&t.d_items_p, items_p.size()));
     HCK(c, dalloc(&t.d_start, start.size()));
-    if (!items.empty())
+    if (!items.empty()) {
         HCK(c, hipMemcpy(t.d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice));
+        HCK(c, hipMemcpy(t.d_items_p, items_p.data(), items_p.size() * 4, hipMemcpyHostToDevice));
+    }
     HCK(c, hipMemcpy(t.d_start, start.data(), start.size() * 4, hipMemcpyHostToDevice));
     c->items.push_back(t);
     *out = t;
@@ -166,6 +176,7 @@ Tables make_tables(const gb_ctx* c, const ItemTable& it) {
     T.inv = c->d_inv;
     T.c32 = c->d_c32;
     T.items = it.d_items;
+    T.items_p = it.d_items_p;
     T.item_start = it.d_start;
     T.p2bits = c->d_p2bits;
     T.p2_limit = c->p2_limit;
@@ -223,6 +234,27 @@ void add_stats(gb_seg_stats* a, const gb_seg_stats& b) {
     a->kernel_ms += b.kernel_ms;
 }
 
+int ensure_r0(gb_ctx* c, uint64_t n) {
+    if (n <= c->r0_cap) return GB_OK;
+    dfree(c->d_r0);
+    c->d_r0 = nullptr;
+    c->r0_cap = 0;
+    HCK(c, dalloc(&c->d_r0, n));
+    c->r0_cap = n;
+    return GB_OK;
+}
+
+// Residue bookkeeping for a window whose first sieved q is rbase and whose
+// largest q is q_top: every sieving prime any of its tiles may use.
+void set_residues(const gb_ctx* c, WinDesc* wd, uint64_t rbase, uint64_t q_top, uint64_t r0off) {
+    const uint64_t s = sieve_prime_bound(q_top);
+    wd->rbase = rbase;
+    wd->r0off = r0off;
+    wd->np = count_le(c->h_sp, std::min<uint64_t>(s, SP_MAX));
+    wd->pad = 0;
+    wd->pad2 = 0;
+}
+
 int ensure_wins(gb_ctx* c, uint32_t n) {
     if (n <= c->wins_cap) return GB_OK;
     dfree(c->d_wins);
@@ -264,10 +296,11 @@ int build_tables(gb_ctx* c) {
     HCK(c, dalloc(&d_seed_c32, 1024));
     HCK(c, dalloc(&d_cnt, 2));
     HCK(c, launch_seed_primes(s, SEED_LIMIT, d_seedbits));
-    HCK(c, launch_compact_primes(s, d_seedbits, seed_words, SEED_LIMIT, d_seed, d_cnt));
+    HCK(c, launch_compact_primes(s, d_seedbits, seed_words, SEED_LIMIT, d_seed, 1024, d_cnt));
     uint32_t nseed = 0;
     HCK(c, hipMemcpyAsync(&nseed, d_cnt, 4, hipMemcpyDeviceToHost, s));
     HCK(c, hipStreamSynchronize(s));
+    if (nseed == 0 || nseed > 1024) return fail(c, GB_EDEVICE, "seed prime count out of range");
     HCK(c, launch_prime_aux(s, d_seed, nseed, d_seed_inv, d_seed_c32));
 
     Tables seedT{};
@@ -275,6 +308,7 @@ int build_tables(gb_ctx* c) {
     seedT.inv = d_seed_inv;
     seedT.c32 = d_seed_c32;
     seedT.items = nullptr;
+    seedT.items_p = nullptr;
     seedT.item_start = nullptr;
     seedT.p2bits = nullptr;
     seedT.p2_limit = 0;
@@ -286,12 +320,12 @@ int build_tables(gb_ctx* c) {
     const uint64_t sp_words = (sp_bits + 63) / 64;
     uint64_t* d_spbits = nullptr;
     HCK(c, dalloc(&d_spbits, sp_words));
-    HCK(c, launch_sieve_window(s, seedT, 3, 0, sp_bits, PLAIN_TILE_BITS, d_spbits, 0));
-    HCK(c, dalloc(&c->d_sp, 300000));
-    HCK(c, launch_compact_primes(s, d_spbits, (uint32_t)sp_words, SP_MAX - 1, c->d_sp, d_cnt));
+    HCK(c, launch_sieve_window(s, seedT, 3, 0, sp_bits, PLAIN_TILE_BITS, d_spbits, 0, nullptr));
+    HCK(c, dalloc(&c->d_sp, SP_TABLE_CAP));
+    HCK(c, launch_compact_primes(s, d_spbits, (uint32_t)sp_words, SP_MAX - 1, c->d_sp, SP_TABLE_CAP, d_cnt));
     HCK(c, hipMemcpyAsync(&c->nsp, d_cnt, 4, hipMemcpyDeviceToHost, s));
     HCK(c, hipStreamSynchronize(s));
-    if (c->nsp == 0 || c->nsp > 300000) return fail(c, GB_EDEVICE, "prime table build produced a bad count");
+    if (c->nsp == 0 || c->nsp > SP_TABLE_CAP) return fail(c, GB_EDEVICE, "prime table build produced a bad count");
     HCK(c, dalloc(&c->d_inv, c->nsp));
     HCK(c, dalloc(&c->d_c32, c->nsp));
     HCK(c, launch_prime_aux(s, c->d_sp, c->nsp, c->d_inv, c->d_c32));
@@ -318,7 +352,7 @@ int build_tables(gb_ctx* c) {
     Tables T = make_tables(c, it);
     T.p2bits = nullptr;
     T.p2_limit = 0;
-    HCK(c, launch_sieve_window(s, T, 3, 0, p2_bits, PLAIN_TILE_BITS, c->d_p2bits, 0));
+    HCK(c, launch_sieve_window(s, T, 3, 0, p2_bits, PLAIN_TILE_BITS, c->d_p2bits, 0, nullptr));
     HCK(c, hipStreamSynchronize(s));
     return GB_OK;
 }
@@ -347,7 +381,10 @@ int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool 
     if (rc) return rc;
     rc = ensure_wins(c, 1);
     if (rc) return rc;
-    WinDesc wd{A, cnt, 0, 0};
+    WinDesc wd{A, cnt, 0, 0, 0, 0, 0, 0, 0};
+    set_residues(c, &wd, g.qv0, geom_q_top(g), 0);
+    rc = ensure_r0(c, wd.np);
+    if (rc) return rc;
     uint64_t lohi[2] = {0, nblocks};
     HCK(c, hipMemcpyAsync(c->d_wins, &wd, sizeof(wd), hipMemcpyHostToDevice, s));
     HCK(c, hipMemcpyAsync(c->d_rec_lo, &lohi[0], 8, hipMemcpyHostToDevice, s));
@@ -355,7 +392,8 @@ int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool 
     SlowList sl = slow_list(c);
     HCK(c, hipEventRecord(c->ev0, s));
     HCK(c, launch_fill(s, c->d_slow_ctl, 2, 0));
-    HCK(c, launch_sieve_window(s, T, g.qv0, g.lead, g.nbits, PLAIN_TILE_BITS, c->d_window, 0));
+    HCK(c, launch_residues(s, T, c->d_wins, 1, wd.np, c->d_r0));
+    HCK(c, launch_sieve_window(s, T, g.qv0, g.lead, g.nbits, PLAIN_TILE_BITS, c->d_window, 0, c->d_r0));
     uint32_t nrec = 0;
     HCK(c, launch_mark_segment(s, T, mp, A, cnt, c->d_window, wwords, MARK_WPB,
                                want_verified ? c->d_verified : nullptr, want_pmin ? c->d_pmin : nullptr, sl,
@@ -484,11 +522,12 @@ int gb_close(gb_ctx* c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (auto& it : c->items) {
         dfree(it.d_items);
+        dfree(it.d_items_p);
         dfree(it.d_start);
     }
     void* bufs[] = {c->d_sp, c->d_inv, c->d_c32, c->d_p2bits, c->d_window, c->d_verified, c->d_pmin, c->d_rec,
                     c->d_out, c->d_slow_n, c->d_slow_win, c->d_slow_ctl, c->d_slow_res, c->d_wins, c->d_rec_lo,
-                    c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp};
+                    c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0};
     for (void* b : bufs)
         if (b) dfree(b);
     if (c->h_out) dhostfree(c->h_out);
@@ -514,11 +553,20 @@ int gb_sieve_window(gb_ctx* c, uint64_t q_low, uint64_t q_high, uint64_t* bits_o
     const uint64_t nbits = (q_high - q_low) / 2 + 1;
     // stream the window through the segment buffer in word-aligned chunks
     const uint64_t chunk_bits = (c->window_cap - 1) * 64;
+    rc = ensure_wins(c, 1);
+    if (rc) return rc;
     for (uint64_t b0 = 0; b0 < nbits; b0 += chunk_bits) {
         const uint64_t nb = std::min(chunk_bits, nbits - b0);
         const uint64_t words = (nb + 63) / 64;
-        HCK(c, launch_sieve_window(c->stream, T, q_low + 2 * b0, 0, nb, PLAIN_TILE_BITS, c->d_window,
-                                   b0 + nb == nbits ? 1 : 0));
+        const uint64_t qlo = q_low + 2 * b0;
+        WinDesc wd{0, 0, 0, 0, 0, 0, 0, 0, 0};
+        set_residues(c, &wd, qlo, qlo + 2 * (nb - 1), 0);
+        rc = ensure_r0(c, wd.np);
+        if (rc) return rc;
+        HCK(c, hipMemcpyAsync(c->d_wins, &wd, sizeof(wd), hipMemcpyHostToDevice, c->stream));
+        HCK(c, launch_residues(c->stream, T, c->d_wins, 1, wd.np, c->d_r0));
+        HCK(c, launch_sieve_window(c->stream, T, qlo, 0, nb, PLAIN_TILE_BITS, c->d_window,
+                                   b0 + nb == nbits ? 1 : 0, c->d_r0));
         HCK(c, hipMemcpyAsync(bits_out + b0 / 64, c->d_window, words * 8, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
     }
@@ -560,7 +608,8 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
     const uint32_t nwin = (uint32_t)wins_in.size();
     std::vector<WinDesc> wins(wins_in);
     std::vector<uint64_t> lo(nwin), hi(nwin);
-    uint64_t tiles = 0, words = 0;
+    uint64_t tiles = 0, words = 0, nres = 0;
+    uint32_t max_np = 0;
     for (uint32_t w = 0; w < nwin; ++w) {
         wins[w].tile0 = tiles;
         wins[w].word0 = words;
@@ -568,8 +617,19 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
         tiles += (wins[w].evens + te - 1) / te;
         hi[w] = tiles;
         words += (wins[w].evens + 63) / 64;
+        // every tile's first sieved q is >= rbase; the last tile reaches q_top
+        const uint64_t start = wins[w].start;
+        const uint64_t rbase = start >= 2ull * HALO + 4 ? start - 2ull * HALO - 1 : 3;
+        const uint64_t last_n0 = start + 2ull * te * ((wins[w].evens - 1) / te);
+        const uint64_t last_cnt = wins[w].evens - (uint64_t)te * ((wins[w].evens - 1) / te);
+        const WinGeom lg = window_geom(last_n0, last_cnt + HALO + WIN_PAD);
+        set_residues(c, &wins[w], rbase, geom_q_top(lg), nres);
+        nres += wins[w].np;
+        max_np = std::max(max_np, wins[w].np);
     }
     rc = ensure_rec(c, tiles);
+    if (rc) return rc;
+    rc = ensure_r0(c, nres);
     if (rc) return rc;
     rc = ensure_wins(c, nwin);
     if (rc) return rc;
@@ -591,6 +651,7 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
     key.push_back((uint64_t)(uintptr_t)c->d_rec);
     key.push_back((uint64_t)(uintptr_t)c->d_wins);
     key.push_back((uint64_t)(uintptr_t)c->d_out);
+    key.push_back((uint64_t)(uintptr_t)c->d_r0);
     if (!c->graph || key != c->graph_key) {
         if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
         HCK(c, hipStreamSynchronize(s));
@@ -598,9 +659,10 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
         HCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t e = launch_fill(s, c->d_slow_ctl, 2, 0);
         if (e == hipSuccess) e = launch_fill(s, c->d_counter, 1, 0);
+        if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, nwin, max_np, c->d_r0);
         if (e == hipSuccess)
             e = launch_fused(s, T, mp, c->d_wins, nwin, tiles, te, (uint32_t)grid, c->d_counter, verified_dev, sl,
-                             c->d_rec);
+                             c->d_rec, c->d_r0);
         if (e == hipSuccess) e = launch_slow(s, T, mp, sl, c->d_slow_res, c->d_wins, verified_dev, nullptr, 0);
         if (e == hipSuccess)
             e = launch_reduce(s, mp, c->d_rec, c->d_rec_lo, c->d_rec_hi, nwin, sl, c->d_slow_res, c->d_out);
@@ -642,7 +704,7 @@ int gb_verify_range(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small,
     gb_seg_stats tot{};
     if (t.form == GB_FORM_BATCHED) {
         std::vector<WinDesc> w(1);
-        w[0] = WinDesc{start, evens, 0, 0};
+        w[0] = WinDesc{start, evens, 0, 0, 0, 0, 0, 0, 0};
         double ms = 0;
         int rc = run_batched(c, w, p_small, t.flags, t.tile_log2, t.wg_per_cu, nullptr, &ms);
         if (rc) return rc;
@@ -685,7 +747,7 @@ int gb_verify_windows(gb_ctx* c, const gb_window* windows, uint32_t count, uint6
         const uint64_t a = windows[i].start, e = windows[i].evens;
         if (a < 4 || (a & 1) || e == 0 || e - 1 > (~0ull - 1 - a) / 2)
             return fail(c, GB_EINVAL, "window " + std::to_string(i) + " outside [4, 2^64-2] or not even");
-        w[i] = WinDesc{a, e, 0, 0};
+        w[i] = WinDesc{a, e, 0, 0, 0, 0, 0, 0, 0};
     }
     HCK(c, hipSetDevice(c->device));
     double ms = 0;

@@ -32,6 +32,7 @@ struct Tables {
     const double* inv;      // 1.0 / p (rounded)
     const uint32_t* c32;    // 2^32 mod p
     const uint32_t* items;  // medium-prime work items: (prime index << 12) | chunk
+    const uint32_t* items_p;     // the prime of each item (saves a dependent load)
     const uint32_t* item_start;  // item_start[k] = first item of medium prime k
     const uint64_t* p2bits; // odd-only bitset of primes <= p2_limit (bit (q-3)/2)
     uint64_t p2_limit;      // odd upper end of p2bits
@@ -90,6 +91,38 @@ __device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t p, double inv
     r = fma(-q, pd, y);
     r = r < 0.0 ? r + pd : (r >= pd ? r - pd : r);
     return (uint32_t)r;
+}
+
+// (rbase + d) mod p from r0 = rbase mod p, for d < 2^52: y = d + r0 < 2^53 is
+// exact in fp64 and one corrected quotient estimate finishes the reduction.
+__device__ __forceinline__ uint32_t mod_from_base(uint64_t d, uint32_t r0, uint32_t p, double inv) {
+    const double pd = (double)p;
+    const double y = (double)d + (double)r0;
+    const double q = floor(y * inv);
+    double r = fma(-q, pd, y);
+    r = r < 0.0 ? r + pd : (r >= pd ? r - pd : r);
+    return (uint32_t)r;
+}
+
+// (rbase + d) mod p as above with the fp64 value of d precomputed (uniform per
+// tile); the fix-ups run on integers: the estimate leaves r in [-p, 2p).
+__device__ __forceinline__ uint32_t mod_base_u32(double df, uint32_t r0, uint32_t p, double inv) {
+    const double y = df + (double)r0;
+    const double q = floor(y * inv);
+    uint32_t u = (uint32_t)(int32_t)fma(-q, (double)p, y);
+    u = min(u, u + p);   // r < 0: u wrapped high, u + p is the residue
+    u = min(u, u - p);   // r >= p: u - p is the residue; else u - p wraps high
+    return u;
+}
+
+// First bit index >= lb holding an odd multiple of p that is >= p^2, given
+// r = q_lo mod p for the odd q_lo at bit lb.
+__device__ __forceinline__ uint64_t first_mark_r(uint64_t q_lo, uint32_t r, uint32_t lb, uint32_t p) {
+    uint64_t d = r ? (uint64_t)(p - r) : 0;
+    if (d & 1) d += p;                 // q_lo + d odd
+    const uint64_t pp = (uint64_t)p * p;
+    if (q_lo < pp && q_lo + d < pp) return lb + (pp - q_lo) / 2;
+    return lb + d / 2;
 }
 
 // First bit index >= lb holding an odd multiple m of p with m >= p^2, where

@@ -50,6 +50,10 @@ __device__ __forceinline__ uint32_t count_le(const uint32_t* sp, uint32_t n, uin
 
 __device__ __noinline__ bool is_prime_u64_call(uint64_t q) { return is_prime_u64(q); }
 
+// Clear bit i of the LDS sieve window.  One LDS instruction per mark: a CU
+// retires about one wave64 LDS instruction per 4.5 clocks (ds_and_b32 and
+// plain stores alike; ds_mskor_b32 is slower), which with ~5 VALU per mark
+// keeps the two units roughly balanced.
 __device__ __forceinline__ void lds_clear_bit(uint32_t* win32, uint32_t i) {
     atomicAnd(&win32[i >> 5], ~(1u << (i & 31)));
 }
@@ -57,10 +61,13 @@ __device__ __forceinline__ void lds_clear_bit(uint32_t* win32, uint32_t i) {
 // ---------------------------------------------------------------------------
 // sieve_tile: window bits [g0, g0 + nbits) of a window whose bit i (>= lead)
 // holds q = qv0 + 2(i - lead).  Result in win32[0 .. ceil(nbits/32)).
+// r0[k] = rbase mod sp[k] (rbase <= q_lo, q_lo - rbase < 2^52) for the first
+// sieving primes; without it the residues come from two fp64 reductions.
 // ---------------------------------------------------------------------------
 template <int BS>
 __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ med_i0, uint64_t qv0,
-                           uint32_t lead, uint64_t g0, uint32_t nbits, const Tables& T, uint32_t tid) {
+                           uint32_t lead, uint64_t g0, uint32_t nbits, const Tables& T, uint32_t tid,
+                           const uint32_t* __restrict__ r0, uint64_t rbase) {
     uint64_t* win64 = reinterpret_cast<uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
     const uint64_t b_lo = g0 > lead ? g0 : (uint64_t)lead;
@@ -71,6 +78,7 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     const uint64_t s = isqrt64(q_top);
     const bool mr = s > T.cover;
     const uint32_t np = count_le(T.sp, T.nsp, mr ? (uint64_t)MR_PRESIEVE : s);
+    const uint64_t dq = q_lo - rbase;   // < 2^52 whenever r0 is given
 
     // 1) primes 3..61 as per-word masks: bit j of word k is q(64k+j); the
     //    multiples of p in a word sit at j0, j0+p, ... with j0 stepping down by
@@ -79,11 +87,12 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
 #pragma unroll
     for (int i = 0; i < NPAT; ++i) {
         const uint32_t p = pat_p(i);
-        const uint32_t r0 = (uint32_t)((q_lo % p + p - (2ull * lb) % p) % p);   // q at local bit 0, mod p
-        const uint32_t res = (r0 + (128u * tid) % p) % p;
+        const uint32_t r = (uint32_t)((q_lo % p + p - (2ull * lb) % p) % p);   // q at local bit 0, mod p
+        const uint32_t res = (r + (128u * tid) % p) % p;
         const uint32_t neg = res ? p - res : 0;
         j0[i] = (neg * ((p + 1) / 2)) % p;
     }
+    const bool low = g0 < lead;   // only the window's first tile holds q < 3
     for (uint32_t k = tid; k < nw64; k += BS) {
         uint64_t acc = 0;
 #pragma unroll
@@ -94,20 +103,11 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
             j0[i] = t < 0 ? (uint32_t)(t + (int)p) : (uint32_t)t;
         }
         uint64_t w = ~acc;
-        const uint64_t gb = g0 + 64ull * k;
-        if (gb < lead) {
-            const uint64_t zb = lead - gb;
-            w = zb >= 64 ? 0 : (w & (~0ull << zb));
-        }
-        if (any_valid && qv0 <= 61) {
-            // the pattern primes are themselves prime
-#pragma unroll
-            for (int i = 0; i < NPAT; ++i) {
-                const uint32_t p = pat_p(i);
-                if (p >= qv0) {
-                    const uint64_t bi = lead + (p - qv0) / 2;
-                    if (bi >= gb && bi < gb + 64) w |= 1ull << (bi - gb);
-                }
+        if (low) {
+            const uint64_t gb = g0 + 64ull * k;
+            if (gb < lead) {
+                const uint64_t zb = lead - gb;
+                w = zb >= 64 ? 0 : (w & (~0ull << zb));
             }
         }
         win64[k] = w;
@@ -116,30 +116,84 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     // 2) first marks of the medium primes (split into CHUNK-mark items)
     const uint32_t kmed_end = np < T.n_med ? np : T.n_med;
     for (uint32_t k = NPAT + tid; k < kmed_end; k += BS) {
-        uint64_t i0 = any_valid ? first_mark(q_lo, lb, T.sp[k], T.inv[k], T.c32[k]) : nbits;
+        const uint32_t p = T.sp[k];
+        uint64_t i0 = nbits;
+        if (any_valid) {
+            const uint32_t r = r0 ? mod_from_base(dq, r0[k], p, T.inv[k]) : mod_small(q_lo, p, T.inv[k], T.c32[k]);
+            i0 = first_mark_r(q_lo, r, lb, p);
+        }
         med_i0[k - NPAT] = i0 < nbits ? (uint32_t)i0 : nbits;
     }
     __syncthreads();
 
-    // 3) marks: medium items, then one thread per large prime
+    // the pattern primes are themselves prime (marks below never touch q < 67^2)
+    if (any_valid && qv0 <= 61 && tid < NPAT) {
+        const uint32_t p = pat_p(tid);
+        if (p >= qv0) {
+            const uint64_t bi = lead + (p - qv0) / 2;
+            if (bi >= g0 && bi < g0 + nbits) atomicOr(&win32[(bi - g0) >> 5], 1u << ((bi - g0) & 31));
+        }
+    }
+
+    // 3a) medium items: CHUNK marks each (unrolled when the chunk is whole);
+    //     the next item's words are loaded before this item's marks issue
     const uint32_t n_items = kmed_end > NPAT ? T.item_start[kmed_end] : 0;
-    const uint32_t n_large = np > T.n_med ? np - T.n_med : 0;
-    const uint32_t total = n_items + n_large;
-    for (uint32_t u = tid; u < total; u += BS) {
+    {
+        uint32_t u = tid, it_n = 0, p_n = 0;
         if (u < n_items) {
-            const uint32_t it = T.items[u];
-            const uint32_t k = it >> 12, c = it & 4095u;
-            const uint32_t p = T.sp[k];
-            uint32_t i = med_i0[k - NPAT] + c * (CHUNK * p);
-            const uint32_t m_end = i < nbits ? (nbits - 1 - i) / p + 1 : 0;
-            const uint32_t m_cnt = m_end < CHUNK ? m_end : CHUNK;
-            for (uint32_t m = 0; m < m_cnt; ++m, i += p) lds_clear_bit(win32, i);
-        } else if (any_valid) {
-            const uint32_t k = T.n_med + (u - n_items);
-            const uint32_t p = T.sp[k];
-            const uint64_t i0 = first_mark(q_lo, lb, p, T.inv[k], T.c32[k]);
-            if (i0 < nbits)
-                for (uint32_t i = (uint32_t)i0; i < nbits; i += p) lds_clear_bit(win32, i);
+            it_n = T.items[u];
+            p_n = T.items_p[u];
+        }
+        while (u < n_items) {
+            const uint32_t it = it_n, p = p_n, un = u + BS;
+            if (un < n_items) {
+                it_n = T.items[un];
+                p_n = T.items_p[un];
+            }
+            const uint32_t i = med_i0[(it >> 12) - NPAT] + (it & 4095u) * (CHUNK * p);
+            if (i + (CHUNK - 1) * p < nbits) {
+#pragma unroll
+                for (uint32_t m = 0; m < CHUNK; ++m) lds_clear_bit(win32, i + m * p);
+            } else {
+                for (uint32_t m = i; m < nbits; m += p) lds_clear_bit(win32, m);
+            }
+            u = un;
+        }
+    }
+    // 3b) large primes, one per thread; the next prime's table entries are
+    //     loaded before this prime's marks issue.  Primes p <= sqrt(q_lo) need
+    //     no p^2 adjustment (all but the last few of a tile).
+    if (any_valid) {
+        const double dqf = (double)dq;
+        const uint64_t root_lo = isqrt64(q_lo);
+        uint32_t k = T.n_med + tid, p_n = 0, r_n = 0;
+        double inv_n = 0.0;
+        if (k < np) {
+            p_n = T.sp[k];
+            inv_n = T.inv[k];
+            r_n = r0 ? r0[k] : T.c32[k];
+        }
+        while (k < np) {
+            const uint32_t p = p_n, rr = r_n, kn = k + BS;
+            const double iv = inv_n;
+            if (kn < np) {
+                p_n = T.sp[kn];
+                inv_n = T.inv[kn];
+                r_n = r0 ? r0[kn] : T.c32[kn];
+            }
+            const uint32_t r = r0 ? mod_base_u32(dqf, rr, p, iv) : mod_small(q_lo, p, iv, rr);
+            uint32_t i0;
+            if (p <= root_lo) {
+                uint32_t t = p - r;
+                t = t == p ? 0u : t;
+                t += (t & 1u) * p;       // the odd multiple
+                i0 = lb + (t >> 1);
+            } else {
+                const uint64_t f = first_mark_r(q_lo, r, lb, p);
+                i0 = f < nbits ? (uint32_t)f : nbits;
+            }
+            for (uint32_t m = i0; m < nbits; m += p) lds_clear_bit(win32, m);
+            k = kn;
         }
     }
     __syncthreads();
@@ -200,8 +254,7 @@ __device__ __forceinline__ void wave_reduce(LaneStats& s) {
 
 // Block-wide merge of lane statistics into one record (all threads must call).
 template <int BS>
-__device__ void block_store(LaneStats s, uint64_t evens, StatRec* out, uint32_t tid) {
-    __shared__ LaneStats red[BS / 64];
+__device__ void block_store(LaneStats s, uint64_t evens, StatRec* out, uint32_t tid, LaneStats* red) {
     wave_reduce(s);
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
@@ -262,63 +315,206 @@ __device__ __forceinline__ uint64_t funnel_c(uint64_t lo, uint64_t hi) {
     else return (lo >> S) | (hi << (64 - S));
 }
 
-template <int K, bool PM>
-__device__ __forceinline__ void mark_one(const uint64_t* __restrict__ w, uint64_t& v, LaneStats& st, uint64_t nbase) {
+template <int K>
+__device__ __forceinline__ void mark_one(const uint64_t* __restrict__ w, uint64_t& v) {
     if constexpr (K < NMARK) {
         constexpr uint32_t p = kMark.p[K];
         constexpr uint32_t off = HALO - (p - 1) / 2;
-        const uint64_t R = funnel_c<off & 63>(w[off >> 6], w[(off >> 6) + 1]);
-        if constexpr (PM) {
-            const uint64_t nb = R & ~v;
-            if (nb) {
-                st.sum += (uint64_t)__popcll(nb) * p;
-                merge_max(st.mx, st.arg, p, nbase + 2ull * __builtin_ctzll(nb));
-            }
+        v |= funnel_c<off & 63>(w[off >> 6], w[(off >> 6) + 1]);
+    }
+}
+
+template <int K, int... I>
+__device__ __forceinline__ void mark_group(const uint64_t* __restrict__ w, uint64_t& v,
+                                           std::integer_sequence<int, I...>) {
+    (mark_one<K + I>(w, v), ...);
+}
+
+// primes [K, KEND) in groups of 8; with EXIT the wave stops once all 64 words are set.
+// The empty asm between groups keeps each group's window reads inside it (no
+// hoisting of all ~65 words into registers up front).
+template <int K, int KEND, bool EXIT>
+__device__ __forceinline__ void mark_span(const uint64_t* __restrict__ w, uint64_t& v) {
+    if constexpr (K < KEND && K < NMARK) {
+        mark_group<K>(w, v, std::make_integer_sequence<int, 8>{});
+        if constexpr (K + 8 < KEND && K + 8 < NMARK) {
+            asm volatile("" ::: "memory");
+            if (!EXIT || !__all(v == ~0ull)) mark_span<K + 8, KEND, EXIT>(w, v);
         }
-        v |= R;
     }
 }
 
-template <int K, bool PM, int... I>
-__device__ __forceinline__ void mark_group(const uint64_t* __restrict__ w, uint64_t& v, LaneStats& st,
-                                           uint64_t nbase, std::integer_sequence<int, I...>) {
-    (mark_one<K + I, PM>(w, v, st, nbase), ...);
+// Compaction stages: every word runs primes [0, MK1); words still incomplete
+// are packed 64 at a time and run [MK1, MK2); the rest are packed again and
+// run to completion.  A wave then pays for the slowest of 64 similar words
+// instead of the slowest of 64 arbitrary ones.
+constexpr int MK1 = 48;
+constexpr int MK2 = 96;
+
+// Up to 128 pending (word, state) entries, two register slots per lane.
+struct Pending {
+    uint32_t w0, w1;
+    uint64_t v0, v1;
+    uint32_t n;   // wave-uniform count
+};
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// primes K, K+1, ... in groups of 8 until the wave's 64 words are all set
-template <int K, bool PM>
-__device__ __forceinline__ void mark_run(const uint64_t* __restrict__ w, uint64_t& v, LaneStats& st, uint64_t nbase) {
-    mark_group<K, PM>(w, v, st, nbase, std::make_integer_sequence<int, 8>{});
-    if constexpr (K + 8 < NMARK) {
-        // keeps each group's window reads inside the group (no hoisting of
-        // all ~65 words into registers up front)
-        asm volatile("" ::: "memory");
-        if (!__all(v == ~0ull)) mark_run<K + 8, PM>(w, v, st, nbase);
+// Append the lanes with `take` set to P.  One forward permute moves every
+// lane: takers to P.n + rank (mod 64), the others to the lanes after them, so
+// each lane receives exactly one value.
+__device__ __forceinline__ void pend_push(Pending& P, bool take, uint32_t w, uint64_t v, uint32_t lane) {
+    const uint64_t T = __ballot(take);
+    const uint32_t U = __popcll(T);
+    if (U == 0) return;
+    const uint32_t rank = take ? lane_rank(T) : U + lane_rank(~T);
+    const int dst = (int)(((P.n + rank) & 63u) << 2);
+    const uint32_t rw = (uint32_t)__builtin_amdgcn_ds_permute(dst, (int)w);
+    const uint32_t rlo = (uint32_t)__builtin_amdgcn_ds_permute(dst, (int)(uint32_t)v);
+    const uint32_t rhi = (uint32_t)__builtin_amdgcn_ds_permute(dst, (int)(uint32_t)(v >> 32));
+    const uint32_t off = (lane - P.n) & 63u;
+    if (off < U) {
+        const uint64_t rv = ((uint64_t)rhi << 32) | rlo;
+        if (P.n + off < 64) { P.w0 = rw; P.v0 = rv; }
+        else { P.w1 = rw; P.v1 = rv; }
     }
+    P.n += U;
+}
+
+__device__ __forceinline__ void pend_pop(Pending& P) {
+    P.w0 = P.w1;
+    P.v0 = P.v1;
+    P.n -= 64;
 }
 
 // ---------------------------------------------------------------------------
 // mark_words: evens n = n_first + 2j, j in [0, cnt).  win64 bit (HALO + j - (p-1)/2)
 // holds q = n - p.  Lanes own 64-even words; the prime loop is wave-uniform.
-// The unrolled path serves p_small >= HALO_P (every marking prime is a
-// Phase-1 prime); the table-driven loop serves the rest and the per-n p_min
-// output.
+// The compacted unrolled path serves p_small >= HALO_P (every marking prime
+// is a Phase-1 prime) without p_min statistics; the table-driven loop serves
+// the rest (small p_small, the Phase-2 test hook, p_min statistics and the
+// per-n p_min output).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t word_valid(uint32_t w, uint32_t nw, uint32_t cnt) {
+    return (w == nw - 1 && (cnt & 63)) ? ((1ull << (cnt & 63)) - 1) : ~0ull;
+}
+
+// Results of one word whose marking is complete: verified bits, Phase-2
+// count, and the n no window prime resolved (failures, or the slow list).
+__device__ __forceinline__ void finish_word(uint32_t w, uint64_t v1, uint64_t v, uint64_t valid, uint64_t n_first,
+                                            const MarkParams& mp, uint64_t* __restrict__ verified,
+                                            uint32_t* __restrict__ pmin_out, const SlowList& sl, uint32_t win_idx,
+                                            LaneStats& st) {
+    st.ph2 += __popcll(~v1 & valid);
+    uint64_t u = ~v & valid;
+    if (u) {
+        if (mp.fast_final) {
+            st.fail += __popcll(u);
+            const uint64_t n = n_first + 2ull * (64ull * w + __builtin_ctzll(u));
+            st.first = n < st.first ? n : st.first;
+            if (pmin_out)
+                while (u) {
+                    const uint32_t b = __builtin_ctzll(u);
+                    u &= u - 1;
+                    pmin_out[64u * w + b] = 0;
+                }
+        } else {
+            while (u) {
+                const uint32_t b = __builtin_ctzll(u);
+                u &= u - 1;
+                const uint32_t slot = atomicAdd(sl.count, 1u);
+                if (slot < sl.cap) {
+                    sl.n[slot] = n_first + 2ull * (64ull * w + b);
+                    sl.win[slot] = win_idx;
+                } else {
+                    atomicOr(sl.overflow, 1u);
+                }
+            }
+        }
+    }
+    if (verified) verified[w] = v1 & valid;
+}
+
+template <int BS>
+__device__ void mark_words_fast(const uint64_t* __restrict__ win64, uint64_t n_first, uint32_t cnt,
+                                const MarkParams& mp, uint64_t* __restrict__ verified, const SlowList& sl,
+                                uint32_t win_idx, LaneStats& st, uint32_t tid) {
+    const uint32_t lane = tid & 63, wave = tid >> 6;
+    const uint32_t nw = (cnt + 63) >> 6;
+    Pending P1, P2;
+    P1.n = 0; P2.n = 0;
+    P1.w0 = P1.w1 = P2.w0 = P2.w1 = 0;
+    P1.v0 = P1.v1 = P2.v0 = P2.v1 = ~0ull;
+    auto done = [&](uint32_t w, uint64_t v) {
+        finish_word(w, v, v, word_valid(w, nw, cnt), n_first, mp, verified, nullptr, sl, win_idx, st);
+    };
+    // stage B on the 64 entries in P1's first slot; incomplete ones move to P2
+    auto run_b = [&](uint32_t live) {
+        const bool ok = lane < live;
+        const uint32_t w = ok ? P1.w0 : 0;
+        uint64_t v = ok ? P1.v0 : ~0ull;
+        mark_span<MK1, MK2, true>(win64 + w, v);
+        const bool unf = v != ~0ull;
+        if (ok && !unf) done(w, v);
+        pend_push(P2, unf, w, v, lane);
+    };
+    auto run_c = [&](uint32_t live) {
+        const bool ok = lane < live;
+        const uint32_t w = ok ? P2.w0 : 0;
+        uint64_t v = ok ? P2.v0 : ~0ull;
+        mark_span<MK2, NMARK, true>(win64 + w, v);
+        if (ok) done(w, v);
+    };
+    for (uint32_t base = wave * 64; base < nw; base += BS) {
+        const uint32_t w = base + lane;
+        const bool act = w < nw;
+        const uint32_t wr = act ? w : 0;
+        uint64_t v = act ? ~word_valid(w, nw, cnt) : ~0ull;
+        if (act && w == 0 && n_first == 4) v |= 1;   // 4 = 2 + 2, the only partition with p = 2
+        mark_span<0, MK1, false>(win64 + wr, v);
+        const bool unf = v != ~0ull;
+        if (act && !unf) done(w, v);
+        pend_push(P1, unf, wr, v, lane);
+        if (P1.n >= 64) {
+            run_b(64);
+            pend_pop(P1);
+        }
+        if (P2.n >= 64) {
+            run_c(64);
+            pend_pop(P2);
+        }
+    }
+    if (P1.n) {
+        run_b(P1.n);
+        P1.n = 0;
+    }
+    while (P2.n) {
+        const uint32_t live = P2.n < 64 ? P2.n : 64;
+        run_c(live);
+        if (P2.n >= 64) pend_pop(P2);
+        else P2.n = 0;
+    }
+}
+
 template <int BS>
 __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first, uint32_t cnt,
                            const MarkParams& mp, const uint32_t* __restrict__ sp, uint64_t* __restrict__ verified,
                            uint32_t* __restrict__ pmin_out, const SlowList& sl, uint32_t win_idx, LaneStats& st,
                            uint32_t tid) {
+    const bool want = mp.flags & 1u;
+    if (mp.K == NMARK && mp.p_small >= HALO_P && pmin_out == nullptr && !want) {
+        mark_words_fast<BS>(win64, n_first, cnt, mp, verified, sl, win_idx, st, tid);
+        return;
+    }
     const uint32_t lane = tid & 63, wave = tid >> 6;
     const uint32_t nw = (cnt + 63) >> 6;
-    const bool want = mp.flags & 1u;
-    const bool fast = mp.K == NMARK && mp.p_small >= HALO_P && pmin_out == nullptr && !want;
     for (uint32_t base = wave * 64; base < nw; base += BS) {
         const uint32_t w = base + lane;
         const bool act = w < nw;
         const uint32_t wr = act ? w : 0;
-        uint64_t valid = 0;
-        if (act) valid = (w == nw - 1 && (cnt & 63)) ? ((1ull << (cnt & 63)) - 1) : ~0ull;
+        const uint64_t valid = act ? word_valid(w, nw, cnt) : 0;
         uint64_t v = ~valid;
         const bool has4 = act && w == 0 && n_first == 4;
         if (has4 && mp.bound >= 2) {
@@ -331,12 +527,6 @@ __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first,
         }
         uint64_t v1 = ~valid | ((has4 && mp.p_small >= 2) ? 1ull : 0ull);
         bool crossed = mp.p_small < 3;
-        if (fast) {
-            const uint64_t nbase = n_first + 128ull * w;
-            mark_run<0, false>(win64 + wr, v, st, nbase);
-            v1 = v;
-            crossed = true;
-        } else
         for (uint32_t k = 0; k < mp.K; ++k) {
             const uint32_t p = sp[k];
             if (!crossed && p > mp.p_small) {
@@ -363,35 +553,7 @@ __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first,
             if ((k & 7) == 7 && __all(v == ~0ull)) break;
         }
         if (!crossed) v1 = v;
-        if (!act) continue;
-        st.ph2 += __popcll(~v1 & valid);
-        uint64_t u = ~v & valid;
-        if (u) {
-            if (mp.fast_final) {
-                st.fail += __popcll(u);
-                const uint64_t n = n_first + 2ull * (64ull * w + __builtin_ctzll(u));
-                st.first = n < st.first ? n : st.first;
-                if (pmin_out)
-                    while (u) {
-                        const uint32_t b = __builtin_ctzll(u);
-                        u &= u - 1;
-                        pmin_out[64u * w + b] = 0;
-                    }
-            } else {
-                while (u) {
-                    const uint32_t b = __builtin_ctzll(u);
-                    u &= u - 1;
-                    const uint32_t slot = atomicAdd(sl.count, 1u);
-                    if (slot < sl.cap) {
-                        sl.n[slot] = n_first + 2ull * (64ull * w + b);
-                        sl.win[slot] = win_idx;
-                    } else {
-                        atomicOr(sl.overflow, 1u);
-                    }
-                }
-            }
-        }
-        if (verified) verified[w] = v1 & valid;
+        if (act) finish_word(w, v1, v, valid, n_first, mp, verified, pmin_out, sl, win_idx, st);
     }
 }
 
@@ -401,7 +563,7 @@ __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first,
 template <int BS>
 __global__ __launch_bounds__(BS) void sieve_window_kernel(Tables T, uint64_t qv0, uint32_t lead, uint64_t nbits,
                                                           uint32_t tile_bits, uint64_t* __restrict__ out,
-                                                          int tail_ones) {
+                                                          int tail_ones, const uint32_t* __restrict__ r0) {
     extern __shared__ uint32_t lds[];
     uint32_t* win32 = lds;
     uint32_t* med_i0 = lds + tile_bits / 32 + 2;
@@ -409,7 +571,7 @@ __global__ __launch_bounds__(BS) void sieve_window_kernel(Tables T, uint64_t qv0
     const uint64_t g0 = (uint64_t)blockIdx.x * tile_bits;
     if (g0 >= nbits) return;
     const uint32_t tb = (uint32_t)(nbits - g0 < tile_bits ? nbits - g0 : tile_bits);
-    sieve_tile<BS>(win32, med_i0, qv0, lead, g0, tb, T, tid);
+    sieve_tile<BS>(win32, med_i0, qv0, lead, g0, tb, T, tid, r0, qv0);
     const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
     const uint32_t nw = (tb + 63) >> 6;
     const uint64_t wo = g0 >> 6;
@@ -435,6 +597,7 @@ __global__ __launch_bounds__(BS) void mark_segment_kernel(Tables T, MarkParams m
                                                           StatRec* __restrict__ rec) {
     extern __shared__ uint32_t lds[];
     uint64_t* win64 = reinterpret_cast<uint64_t*>(lds);
+    LaneStats* red = reinterpret_cast<LaneStats*>(lds + 2 * (wpb + HALO / 64 + 1));
     const uint32_t tid = threadIdx.x;
     const uint64_t w0 = (uint64_t)blockIdx.x * wpb;          // first verified word
     const uint64_t j0 = w0 * 64;
@@ -449,7 +612,7 @@ __global__ __launch_bounds__(BS) void mark_segment_kernel(Tables T, MarkParams m
     LaneStats st = lane_stats_zero();
     mark_words<BS>(win64, n0 + 2 * j0, lcnt, mp, T.sp, verified ? verified + w0 : nullptr,
                    pmin_out ? pmin_out + j0 : nullptr, sl, 0, st, tid);
-    block_store<BS>(st, lcnt, &rec[blockIdx.x], tid);
+    block_store<BS>(st, lcnt, &rec[blockIdx.x], tid, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -461,12 +624,15 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
                                                    uint32_t nwin, uint64_t ntiles, uint32_t tile_evens,
                                                    uint64_t* __restrict__ tile_counter,
                                                    uint64_t* __restrict__ verified, SlowList sl,
-                                                   StatRec* __restrict__ rec) {
+                                                   StatRec* __restrict__ rec, const uint32_t* __restrict__ r0) {
     extern __shared__ uint32_t lds[];
+    // dynamic LDS only (window at offset 0): [window | medium first marks | stats | tile id]
     const uint32_t win_words32 = (tile_evens + HALO + WIN_PAD) / 32 + 2;
+    const uint32_t med_words = ((T.n_med > NPAT ? T.n_med - NPAT : 1) + 1) & ~1u;
     uint32_t* win32 = lds;
     uint32_t* med_i0 = lds + win_words32;
-    __shared__ uint64_t s_tile;
+    LaneStats* red = reinterpret_cast<LaneStats*>(lds + win_words32 + med_words);
+    uint64_t& s_tile = *reinterpret_cast<uint64_t*>(red + BS / 64);
     const uint32_t tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s_tile = atomicAdd((unsigned long long*)tile_counter, 1ull);
@@ -486,12 +652,12 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
         const uint64_t rem = wd.evens - (uint64_t)tile_evens * lt;
         const uint32_t cnt = (uint32_t)(rem < tile_evens ? rem : tile_evens);
         const WinGeom g = window_geom(n0, (uint64_t)cnt + HALO + WIN_PAD);
-        sieve_tile<BS>(win32, med_i0, g.qv0, g.lead, 0, g.nbits, T, tid);
+        sieve_tile<BS>(win32, med_i0, g.qv0, g.lead, 0, g.nbits, T, tid, r0 ? r0 + wd.r0off : nullptr, wd.rbase);
         LaneStats st = lane_stats_zero();
         mark_words<BS>(reinterpret_cast<const uint64_t*>(win32), n0, cnt, mp, T.sp,
                        verified ? verified + wd.word0 + lt * (tile_evens / 64) : nullptr, nullptr, sl, lo, st,
                        tid);
-        block_store<BS>(st, cnt, &rec[t], tid);
+        block_store<BS>(st, cnt, &rec[t], tid, red);
     }
 }
 
@@ -592,6 +758,13 @@ __global__ __launch_bounds__(256) void reduce_kernel(MarkParams mp, const StatRe
     }
 }
 
+// r0[wins[w].r0off + k] = wins[w].rbase mod sp[k], k < wins[w].np (grid.y = window)
+__global__ void residue_kernel(Tables T, const WinDesc* __restrict__ wins, uint32_t* __restrict__ r0) {
+    const WinDesc wd = wins[blockIdx.y];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < wd.np; k += gridDim.x * blockDim.x)
+        r0[wd.r0off + k] = mod_small(wd.rbase, T.sp[k], T.inv[k], T.c32[k]);
+}
+
 __global__ void fill_kernel(uint64_t* p, uint64_t words, uint64_t value) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = value;
@@ -621,7 +794,7 @@ __global__ void seed_primes_kernel(uint32_t limit, uint64_t* bits_out) {
 // single workgroup: list of odd q <= qlimit whose bit is set (bit i <-> 2i+3)
 __global__ __launch_bounds__(1024) void compact_primes_kernel(const uint64_t* __restrict__ bits, uint32_t nwords,
                                                               uint32_t qlimit, uint32_t* __restrict__ out,
-                                                              uint32_t* __restrict__ count_out) {
+                                                              uint32_t cap, uint32_t* __restrict__ count_out) {
     __shared__ uint32_t cnt[1024];
     const uint32_t tid = threadIdx.x;
     const uint32_t per = (nwords + 1023) / 1024;
@@ -648,7 +821,8 @@ __global__ __launch_bounds__(1024) void compact_primes_kernel(const uint64_t* __
         while (x) {
             const uint32_t bb = __builtin_ctzll(x);
             x &= x - 1;
-            out[pos++] = 2 * (64 * w + bb) + 3;
+            if (pos < cap) out[pos] = 2 * (64 * w + bb) + 3;
+            ++pos;
         }
     }
     if (tid == 1023) *count_out = cnt[1023];
@@ -746,7 +920,7 @@ static hipError_t allow_lds(K kernel, size_t bytes) {
 }
 
 hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uint32_t lead, uint64_t nbits,
-                               uint32_t tile_bits, uint64_t* out, int tail_ones) {
+                               uint32_t tile_bits, uint64_t* out, int tail_ones, const uint32_t* r0) {
     if (nbits == 0) return hipSuccess;
     const uint64_t ntiles = (nbits + tile_bits - 1) / tile_bits;
     if (ntiles > 0x7FFFFFFFull) return hipErrorInvalidValue;
@@ -754,7 +928,7 @@ hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uin
     hipError_t e = allow_lds(sieve_window_kernel<SIEVE_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sieve_window_kernel<SIEVE_BS>, dim3((uint32_t)ntiles), dim3(SIEVE_BS), lds, s, T, qv0, lead,
-                       nbits, tile_bits, out, tail_ones);
+                       nbits, tile_bits, out, tail_ones, r0);
     return hipGetLastError();
 }
 
@@ -764,7 +938,7 @@ hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams&
     const uint64_t nw = (cnt + 63) / 64;
     const uint64_t blocks = (nw + wpb - 1) / wpb;
     if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    const size_t lds = (size_t)(wpb + HALO / 64 + 1) * 8;
+    const size_t lds = (size_t)(wpb + HALO / 64 + 1) * 8 + sizeof(LaneStats) * (MARK_BS / 64);
     hipError_t e = allow_lds(mark_segment_kernel<MARK_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mark_segment_kernel<MARK_BS>, dim3((uint32_t)blocks), dim3(MARK_BS), lds, s, T, mp, n0, cnt,
@@ -773,15 +947,25 @@ hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams&
     return hipGetLastError();
 }
 
+hipError_t launch_residues(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin, uint32_t max_np,
+                           uint32_t* r0) {
+    if (!nwin || !max_np) return hipSuccess;
+    uint32_t bx = (max_np + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    hipLaunchKernelGGL(residue_kernel, dim3(bx, nwin), dim3(256), 0, s, T, wins, r0);
+    return hipGetLastError();
+}
+
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
-                        uint64_t* verified, SlowList sl, StatRec* rec) {
+                        uint64_t* verified, SlowList sl, StatRec* rec, const uint32_t* r0) {
     const size_t lds = (size_t)((tile_evens + HALO + WIN_PAD) / 32 + 2) * 4 +
-                       (size_t)(T.n_med > NPAT ? T.n_med - NPAT : 1) * 4;
+                       (size_t)(((T.n_med > NPAT ? T.n_med - NPAT : 1) + 1) & ~1u) * 4 +
+                       sizeof(LaneStats) * (FUSED_BS / 64) + 8;
     hipError_t e = allow_lds(fused_kernel<FUSED_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fused_kernel<FUSED_BS>, dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin, ntiles,
-                       tile_evens, tile_counter, verified, sl, rec);
+                       tile_evens, tile_counter, verified, sl, rec, r0);
     return hipGetLastError();
 }
 
@@ -813,8 +997,9 @@ hipError_t launch_seed_primes(hipStream_t s, uint32_t limit, uint64_t* bits_out)
 }
 
 hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t nwords, uint32_t qlimit,
-                                 uint32_t* primes_out, uint32_t* count_out) {
-    hipLaunchKernelGGL(compact_primes_kernel, dim3(1), dim3(1024), 0, s, bits, nwords, qlimit, primes_out, count_out);
+                                 uint32_t* primes_out, uint32_t cap, uint32_t* count_out) {
+    hipLaunchKernelGGL(compact_primes_kernel, dim3(1), dim3(1024), 0, s, bits, nwords, qlimit, primes_out, cap,
+                       count_out);
     return hipGetLastError();
 }
 

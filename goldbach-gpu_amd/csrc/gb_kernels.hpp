@@ -38,13 +38,36 @@ struct WinDesc {
     uint64_t evens;
     uint64_t tile0;       // global index of the window's first tile
     uint64_t word0;       // first word of the window in the optional verified map
+    uint64_t rbase;       // odd q every tile's q_lo is measured from (q_lo - rbase < 2^52)
+    uint64_t r0off;       // first entry of this window's residues (rbase mod sp[k])
+    uint32_t np;          // residues held: sieving primes any tile of the window uses
+    uint32_t pad;
+    uint64_t pad2;
 };
+
+// Sieving primes needed for windows whose largest q is q_top.
+__host__ __device__ inline uint64_t sieve_prime_bound(uint64_t q_top) {
+    uint64_t r = 0;
+    // floor(sqrt(q_top)) by bisection on 32-bit candidates (host and device)
+    uint64_t lo = 0, hi = 0xFFFFFFFFull;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo + 1) / 2;
+        if (mid <= q_top / mid) lo = mid; else hi = mid - 1;
+    }
+    r = lo;
+    return r;
+}
 
 // ---- launchers (stream-ordered, no synchronisation) ----
 
 // Odd-only sieve of window bits [0, nbits): bit i <-> q = qv0 + 2(i - lead), i >= lead.
+// r0 (nullable): qv0 mod sp[k] for every sieving prime the window needs.
 hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uint32_t lead, uint64_t nbits,
-                               uint32_t tile_bits, uint64_t* out, int tail_ones);
+                               uint32_t tile_bits, uint64_t* out, int tail_ones, const uint32_t* r0);
+
+// r0[wins[w].r0off + k] = wins[w].rbase mod sp[k] for k < wins[w].np.
+hipError_t launch_residues(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin, uint32_t max_np,
+                           uint32_t* r0);
 
 // Plain marking of one segment from a window already in device memory.
 hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams& mp, uint64_t n0, uint64_t cnt,
@@ -55,7 +78,7 @@ hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams&
 // Fused persistent kernel over the tiles of `nwin` windows.
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
-                        uint64_t* verified, SlowList sl, StatRec* rec);
+                        uint64_t* verified, SlowList sl, StatRec* rec, const uint32_t* r0);
 
 // Resolve slow-list entries: search primes above sp[K-1] up to bound.
 hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
@@ -72,7 +95,7 @@ hipError_t launch_fill(hipStream_t s, uint64_t* p, uint64_t words, uint64_t valu
 // Table builders.
 hipError_t launch_seed_primes(hipStream_t s, uint32_t limit, uint64_t* bits_out);
 hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t nwords, uint32_t qlimit,
-                                 uint32_t* primes_out, uint32_t* count_out);
+                                 uint32_t* primes_out, uint32_t cap, uint32_t* count_out);
 hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32);
 
 // Serving: p_out[i] = minimal p for ns[i].
