@@ -107,7 +107,20 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build verifies on one GPU; run with --gpus 1 (multi-device sharding "
                                    "is a design note, not built)"}))
@@ -145,7 +158,7 @@ def main() -> int:
         plain = {"value": evens * psteps / pdt, "ms_per_step": pdt * 1e3 / psteps,
                  "kernel_ms": float(np.mean([o["kernel_ms"] for o in pouts])), "seg_evens": PLAIN_SEG}
 
-    serve = serving(ctx)
+    serve = None if args.no_serve else serving(ctx)
     ctx.close()
     cpu = None if args.no_cpu else cpu_baseline()
 
