@@ -12,6 +12,8 @@ from __future__ import annotations
 
 import ctypes
 import os
+import time
+import weakref
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -32,6 +34,8 @@ EXPORTS = (
     "gb_open", "gb_close", "gb_last_error", "gb_device_count", "gb_device_info",
     "gb_sieve_window", "gb_verify_segment", "gb_verify_range", "gb_verify_windows",
     "gb_min_partition_batch",
+    "gb_serve_open", "gb_serve_submit", "gb_serve_poll", "gb_serve_stats_get", "gb_serve_close",
+    "gb_serve_last_error",
 )
 
 u64 = ctypes.c_uint64
@@ -57,6 +61,23 @@ class GbTuning(ctypes.Structure):
 
 class GbWindow(ctypes.Structure):
     _fields_ = [("start", u64), ("evens", u64)]
+
+
+class GbServeConfig(ctypes.Structure):
+    _fields_ = [("slots", u32), ("chunks_per_step", u32), ("mem_budget", u64), ("queue_cap", u64)]
+
+
+class GbServeStats(ctypes.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("steps", u64), ("step_queries", u64),
+                ("max_resident", u64), ("queued", u64), ("resident", u64), ("slots", u32), ("reserved", u32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+# gb_result as a numpy record (48 bytes, same field order and widths)
+RESULT_DTYPE = np.dtype([("id", "<u8"), ("n", "<u8"), ("p", "<u8"), ("submit_ns", "<u8"), ("done_ns", "<u8"),
+                         ("steps", "<u4"), ("reserved", "<u4")])
 
 
 class GoldbachError(RuntimeError):
@@ -92,6 +113,12 @@ def library() -> ctypes.CDLL:
         "gb_verify_windows": (ctypes.c_int, [vp, ctypes.POINTER(GbWindow), u32, u64, u32,
                                              ctypes.POINTER(GbSegStats), ctypes.POINTER(GbSegStats)]),
         "gb_min_partition_batch": (ctypes.c_int, [vp, vp, u32, vp]),
+        "gb_serve_open": (ctypes.c_int, [vp, ctypes.POINTER(GbServeConfig), ctypes.POINTER(vp)]),
+        "gb_serve_submit": (ctypes.c_int, [vp, vp, u32, ctypes.POINTER(u64)]),
+        "gb_serve_poll": (ctypes.c_int, [vp, vp, u32, ctypes.POINTER(u32), u32]),
+        "gb_serve_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(GbServeStats)]),
+        "gb_serve_close": (ctypes.c_int, [vp]),
+        "gb_serve_last_error": (ctypes.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -130,9 +157,12 @@ class Context:
         if rc != GB_OK:
             raise GoldbachError(rc, self._lib.gb_last_error(None).decode())
         self._h = h
+        self._servers = weakref.WeakSet()
 
     # -- lifecycle -------------------------------------------------------
     def close(self) -> None:
+        for srv in list(getattr(self, "_servers", ())):
+            srv.close()
         if getattr(self, "_h", None):
             self._lib.gb_close(self._h)
             self._h = None
@@ -194,6 +224,90 @@ class Context:
         n = np.ascontiguousarray(np.asarray(ns, dtype=np.uint64))
         out = np.zeros(n.shape[0], dtype=np.uint64)
         self._check(self._lib.gb_min_partition_batch(self._h, _ptr(n), n.shape[0], _ptr(out)))
+        return out
+
+    def server(self, slots: int = 0, chunks_per_step: int = 0, mem_budget: int = 0, queue_cap: int = 0) -> "Server":
+        return Server(self, slots, chunks_per_step, mem_budget, queue_cap)
+
+
+class Server:
+    """Continuous-batching serving engine on a context (gb_serve_*)."""
+
+    def __init__(self, ctx: Context, slots: int = 0, chunks_per_step: int = 0, mem_budget: int = 0,
+                 queue_cap: int = 0):
+        self._lib = ctx._lib
+        self._ctx = ctx
+        cfg = GbServeConfig(slots, chunks_per_step, mem_budget, queue_cap)
+        h = ctypes.c_void_p()
+        ctx._check(self._lib.gb_serve_open(ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._buf = np.zeros(65536, dtype=RESULT_DTYPE)
+        self._stash = np.zeros(0, dtype=RESULT_DTYPE)   # results run() saw for other submissions
+        ctx._servers.add(self)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.gb_serve_close(self._h)
+            self._h = None
+            self._ctx._servers.discard(self)
+
+    def __enter__(self) -> "Server":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != GB_OK:
+            raise GoldbachError(rc, self._lib.gb_serve_last_error(self._h).decode())
+
+    def submit(self, ns) -> int:
+        """Queue even n >= 4; returns the id of the first (the rest follow in order)."""
+        n = np.ascontiguousarray(np.asarray(ns, dtype=np.uint64).reshape(-1))
+        first = u64(0)
+        self._check(self._lib.gb_serve_submit(self._h, _ptr(n), n.shape[0], ctypes.byref(first)))
+        return first.value
+
+    def poll(self, max_results: int = 65536, wait_ms: int = 0) -> np.ndarray:
+        """Finished results (RESULT_DTYPE records) in completion order."""
+        if self._stash.shape[0]:
+            out, self._stash = self._stash[:max_results], self._stash[max_results:]
+            return out
+        return self._poll(max_results, wait_ms)
+
+    def _poll(self, max_results: int, wait_ms: int) -> np.ndarray:
+        if max_results > self._buf.shape[0]:
+            self._buf = np.zeros(max_results, dtype=RESULT_DTYPE)
+        got = u32(0)
+        self._check(self._lib.gb_serve_poll(self._h, _ptr(self._buf), max_results, ctypes.byref(got), wait_ms))
+        return self._buf[:got.value].copy()
+
+    def stats(self) -> dict:
+        st = GbServeStats()
+        self._check(self._lib.gb_serve_stats_get(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def run(self, ns, timeout_s: float = 60.0) -> np.ndarray:
+        """Submit ns and wait for all of them; returns p in submission order."""
+        n = np.asarray(ns, dtype=np.uint64).reshape(-1)
+        first = self.submit(n)
+        out = np.zeros(n.shape[0], dtype=np.uint64)
+        left = n.shape[0]
+        t_end = time.monotonic() + timeout_s
+        while left:
+            r = self._poll(65536, wait_ms=50)
+            mine = (r["id"] >= first) & (r["id"] < first + n.shape[0])
+            out[(r["id"][mine] - first).astype(np.int64)] = r["p"][mine]
+            self._stash = np.concatenate([self._stash, r[~mine]])
+            left -= int(mine.sum())
+            if time.monotonic() > t_end:
+                raise TimeoutError(f"serving: {left} queries unfinished after {timeout_s} s")
         return out
 
 

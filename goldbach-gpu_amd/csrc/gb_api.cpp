@@ -16,17 +16,16 @@
 #include <string>
 #include <vector>
 
-#include "goldbach_gpu.h"
-#include "gb_kernels.hpp"
+#include "gb_ctx.hpp"
 
 using namespace gbk;
+using namespace gbh;
 
 namespace {
 
 constexpr uint64_t DEFAULT_SEG = 1ull << 27;
 constexpr uint64_t DEFAULT_P2 = 100000000ull;           // goldbach.cu:316
 constexpr uint64_t VRAM_MARGIN = 50ull << 20;            // goldbach.cu:96
-constexpr uint32_t PLAIN_TILE_BITS = 1u << 18;          // plain sieve tile (bits per workgroup)
 constexpr uint32_t MARK_WPB = 2048;                      // plain marking: verified words per workgroup
 constexpr uint32_t DEFAULT_TILE_LOG2 = 20;               // fused tile: 2^20 evens
 constexpr uint32_t SLOW_CAP = 1u << 20;
@@ -36,84 +35,6 @@ constexpr uint32_t SP_TABLE_CAP = 300000;                // odd primes below 2^2
 std::mutex g_open_mu;
 std::string g_open_err;
 
-struct ItemTable {
-    uint32_t tb = 0;        // tile bits the schedule was cut for
-    uint32_t n_med = 0;
-    uint32_t* d_items = nullptr;
-    uint32_t* d_items_p = nullptr;
-    uint32_t* d_start = nullptr;
-};
-
-}  // namespace
-
-struct gb_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    gb_config cfg{};
-    int cus = 0;
-    uint64_t total_mem = 0;
-    // prime tables
-    uint32_t* d_sp = nullptr;
-    double* d_inv = nullptr;
-    uint32_t* d_c32 = nullptr;
-    uint32_t nsp = 0;
-    std::vector<uint32_t> h_sp;
-    uint64_t* d_p2bits = nullptr;
-    uint64_t p2_limit = 0;   // odd
-    std::vector<ItemTable> items;
-    // plain segment buffers
-    uint64_t* d_window = nullptr;
-    uint64_t window_cap = 0;   // words
-    uint64_t* d_verified = nullptr;
-    uint64_t verified_cap = 0; // words
-    uint32_t* d_pmin = nullptr;
-    uint64_t pmin_cap = 0;
-    StatRec* d_rec = nullptr;
-    uint64_t rec_cap = 0;
-    StatRec* d_out = nullptr;
-    uint32_t out_cap = 0;
-    // slow list
-    uint64_t* d_slow_n = nullptr;
-    uint32_t* d_slow_win = nullptr;
-    uint64_t* d_slow_ctl = nullptr;   // [0]: count (u32) | overflow (u32) as two words
-    uint64_t* d_slow_res = nullptr;
-    // windows + reduce ranges
-    WinDesc* d_wins = nullptr;
-    uint64_t* d_rec_lo = nullptr;
-    uint64_t* d_rec_hi = nullptr;
-    uint32_t wins_cap = 0;
-    uint64_t* d_counter = nullptr;
-    uint32_t* d_r0 = nullptr;          // per-window residues rbase mod sp[k]
-    uint64_t r0_cap = 0;
-    // serving buffers
-    uint64_t* d_qn = nullptr;
-    uint64_t* d_qp = nullptr;
-    uint32_t q_cap = 0;
-    // host staging (pinned)
-    StatRec* h_out = nullptr;
-    uint64_t* h_ctl = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // cached batched graph
-    hipGraphExec_t graph = nullptr;
-    std::vector<uint64_t> graph_key;
-};
-
-namespace {
-
-int fail(gb_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
-    return code;
-}
-
-#define HCK(ctx, call)                                                                         \
-    do {                                                                                       \
-        hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return fail(ctx, GB_EDEVICE, std::string("HIP error at ") + __FILE__ + ":" +       \
-                                             std::to_string(__LINE__) + " -- " + hipGetErrorString(e_)); \
-    } while (0)
-
 void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
@@ -122,15 +43,17 @@ void dhostfree(void* p) {
     if (p) (void)hipHostFree(p);
 }
 
-template <typename T>
-hipError_t dalloc(T** p, uint64_t count) {
-    *p = nullptr;
-    if (!count) count = 1;
-    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
-}
-
 uint32_t count_le(const std::vector<uint32_t>& sp, uint64_t x) {
     return (uint32_t)(std::upper_bound(sp.begin(), sp.end(), (uint32_t)std::min<uint64_t>(x, 0xFFFFFFFFull)) - sp.begin());
+}
+
+}  // namespace
+
+namespace gbh {
+
+int fail(gb_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
 }
 
 // Medium-prime schedule for tiles of at most `tb` bits: prime k with
@@ -185,6 +108,10 @@ Tables make_tables(const gb_ctx* c, const ItemTable& it) {
     T.n_med = it.n_med;
     return T;
 }
+
+}  // namespace gbh
+
+namespace {
 
 SlowList slow_list(const gb_ctx* c) {
     SlowList s;
@@ -517,6 +444,7 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
 
 int gb_close(gb_ctx* c) {
     if (!c) return GB_OK;
+    if (c->servers.load() > 0) return fail(c, GB_EAGAIN, "gb_close: close the context's servers first");
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);

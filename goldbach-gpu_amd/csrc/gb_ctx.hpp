@@ -1,0 +1,109 @@
+// gb_ctx.hpp -- internal state of a gb_ctx and the helpers the C-ABI
+// translation units share (range verification in gb_api.cpp, the serving
+// engine in gb_serve.cpp).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <string>
+#include <vector>
+
+#include "gb_kernels.hpp"
+#include "goldbach_gpu.h"
+
+namespace gbh {
+
+constexpr uint32_t PLAIN_TILE_BITS = 1u << 18;   // plain sieve tile (bits per workgroup)
+
+struct ItemTable {
+    uint32_t tb = 0;        // tile bits the schedule was cut for
+    uint32_t n_med = 0;
+    uint32_t* d_items = nullptr;
+    uint32_t* d_items_p = nullptr;
+    uint32_t* d_start = nullptr;
+};
+
+}  // namespace gbh
+
+struct gb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    gb_config cfg{};
+    int cus = 0;
+    uint64_t total_mem = 0;
+    // prime tables
+    uint32_t* d_sp = nullptr;
+    double* d_inv = nullptr;
+    uint32_t* d_c32 = nullptr;
+    uint32_t nsp = 0;
+    std::vector<uint32_t> h_sp;
+    uint64_t* d_p2bits = nullptr;
+    uint64_t p2_limit = 0;   // odd
+    std::vector<gbh::ItemTable> items;
+    // plain segment buffers
+    uint64_t* d_window = nullptr;
+    uint64_t window_cap = 0;   // words
+    uint64_t* d_verified = nullptr;
+    uint64_t verified_cap = 0; // words
+    uint32_t* d_pmin = nullptr;
+    uint64_t pmin_cap = 0;
+    gbk::StatRec* d_rec = nullptr;
+    uint64_t rec_cap = 0;
+    gbk::StatRec* d_out = nullptr;
+    uint32_t out_cap = 0;
+    // slow list
+    uint64_t* d_slow_n = nullptr;
+    uint32_t* d_slow_win = nullptr;
+    uint64_t* d_slow_ctl = nullptr;   // [0]: count (u32) | overflow (u32) as two words
+    uint64_t* d_slow_res = nullptr;
+    // windows + reduce ranges
+    gbk::WinDesc* d_wins = nullptr;
+    uint64_t* d_rec_lo = nullptr;
+    uint64_t* d_rec_hi = nullptr;
+    uint32_t wins_cap = 0;
+    uint64_t* d_counter = nullptr;
+    uint32_t* d_r0 = nullptr;          // per-window residues rbase mod sp[k]
+    uint64_t r0_cap = 0;
+    // serving buffers
+    uint64_t* d_qn = nullptr;
+    uint64_t* d_qp = nullptr;
+    uint32_t q_cap = 0;
+    // host staging (pinned)
+    gbk::StatRec* h_out = nullptr;
+    uint64_t* h_ctl = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // cached batched graph
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graph_key;
+    // serving engines running on this context's tables
+    std::atomic<int> servers{0};
+};
+
+namespace gbh {
+
+// Records msg as the context's last error and returns code.
+int fail(gb_ctx* c, int code, const std::string& msg);
+
+#define HCK(ctx, call)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(ctx, GB_EDEVICE, std::string("HIP error at ") + __FILE__ + ":" +       \
+                                             std::to_string(__LINE__) + " -- " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, uint64_t count) {
+    *p = nullptr;
+    if (!count) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+// Medium-prime schedule of the sieve for tiles of at most `tb` bits (cached on c).
+int build_items(gb_ctx* c, uint32_t tb, ItemTable* out);
+// Device table view of c with the schedule `it`.
+gbk::Tables make_tables(const gb_ctx* c, const ItemTable& it);
+
+}  // namespace gbh

@@ -847,6 +847,59 @@ __device__ __forceinline__ bool small_factor(uint64_t q) {
            q % 19 == 0 || q % 23 == 0;
 }
 
+// One 64-candidate chunk of the minimal-partition scan for even n > 4: lane l
+// tries the (base + l)-th odd prime (odd numbers tested by Miller-Rabin past
+// the table).  Returns 1 with the minimal p of the chunk in res, -1 when no
+// candidate is in range any more (no partition up to bound: res = 0), 0 when
+// the chunk holds none and the scan goes on.  Wave-uniform.
+__device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint64_t bound, uint64_t base,
+                               uint32_t lane, uint64_t& res) {
+    const uint64_t k = base + lane;
+    uint64_t p;
+    bool isp;
+    if (k < T.nsp) {
+        p = T.sp[k];
+        isp = true;
+    } else {
+        p = (uint64_t)T.sp[T.nsp - 1] + 2 * (k - T.nsp + 1);
+        isp = p <= half && p <= bound && is_prime_u64(p);
+    }
+    const bool in = p <= half && p <= bound;
+    if (__ballot(in) == 0) {
+        res = 0;
+        return -1;
+    }
+    const uint64_t q = n - p;
+    bool pass = false;
+    if (in && isp) {
+        if (q <= T.p2_limit) {
+            pass = is_prime_q(q, T);
+        } else if (!small_factor(q)) {
+            const Mont M = mont_setup(q);
+            const uint32_t r = __builtin_ctzll(q - 1);
+            pass = sprp(2, (q - 1) >> r, r, M);
+        }
+    }
+    uint64_t ball = __ballot(pass);
+    while (ball) {
+        const uint32_t L = __ffsll((unsigned long long)ball) - 1;
+        const uint64_t qq = __shfl(q, L);
+        bool ok = true;
+        if (qq > T.p2_limit && lane < 12) {
+            const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+            const Mont M = mont_setup(qq);
+            const uint32_t r = __builtin_ctzll(qq - 1);
+            ok = sprp(bases[lane], (qq - 1) >> r, r, M);
+        }
+        if (__all(ok)) {
+            res = __shfl(p, L);
+            return 1;
+        }
+        ball &= ball - 1;
+    }
+    return 0;
+}
+
 __global__ __launch_bounds__(64 * SERVE_WAVES) void min_partition_kernel(Tables T, const uint64_t* __restrict__ ns,
                                                                          uint32_t count, uint64_t bound,
                                                                          uint64_t* __restrict__ p_out) {
@@ -859,54 +912,45 @@ __global__ __launch_bounds__(64 * SERVE_WAVES) void min_partition_kernel(Tables 
         if (n == 4) {
             res = bound >= 2 ? 2 : 0;
         } else {
-            const uint64_t half = n / 2;
-            bool done = false;
-            for (uint64_t base = 0; !done; base += 64) {
-                const uint64_t k = base + lane;
-                uint64_t p;
-                bool isp;
-                if (k < T.nsp) {
-                    p = T.sp[k];
-                    isp = true;
-                } else {
-                    p = (uint64_t)T.sp[T.nsp - 1] + 2 * (k - T.nsp + 1);
-                    isp = p <= half && p <= bound && is_prime_u64(p);
-                }
-                const bool in = p <= half && p <= bound;
-                if (__ballot(in) == 0) break;
-                const uint64_t q = n - p;
-                bool pass = false;
-                if (in && isp) {
-                    if (q <= T.p2_limit) {
-                        pass = is_prime_q(q, T);
-                    } else if (!small_factor(q)) {
-                        const Mont M = mont_setup(q);
-                        const uint32_t r = __builtin_ctzll(q - 1);
-                        pass = sprp(2, (q - 1) >> r, r, M);
-                    }
-                }
-                uint64_t ball = __ballot(pass);
-                while (ball) {
-                    const uint32_t L = __ffsll((unsigned long long)ball) - 1;
-                    const uint64_t qq = __shfl(q, L);
-                    bool ok = true;
-                    if (qq > T.p2_limit && lane < 12) {
-                        const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
-                        const Mont M = mont_setup(qq);
-                        const uint32_t r = __builtin_ctzll(qq - 1);
-                        ok = sprp(bases[lane], (qq - 1) >> r, r, M);
-                    }
-                    if (__all(ok)) {
-                        res = __shfl(p, L);
-                        done = true;
-                        break;
-                    }
-                    ball &= ball - 1;
-                }
-            }
+            for (uint64_t base = 0;; base += 64)
+                if (partition_chunk(T, n, n / 2, bound, base, lane, res) != 0) break;
         }
     }
     if (lane == 0) p_out[qi] = res;
+}
+
+// Continuous-batching step: every admitted query (one wave each, new and
+// in-flight alike) advances by up to `chunks` chunks from its saved position.
+__global__ __launch_bounds__(64 * SERVE_WAVES) void serve_step_kernel(Tables T, const ServeIn* __restrict__ in,
+                                                                      uint32_t count, uint64_t bound, uint32_t chunks,
+                                                                      ServeOut* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t qi = blockIdx.x * SERVE_WAVES + (threadIdx.x >> 6);
+    if (qi >= count) return;   // uniform per wave
+    const uint64_t n = in[qi].n;
+    uint64_t base = in[qi].k, res = 0;
+    uint32_t state = 1;
+    if (n < 4 || (n & 1)) {
+        res = 0;
+    } else if (n == 4) {
+        res = bound >= 2 ? 2 : 0;
+    } else {
+        state = 0;
+        for (uint32_t c = 0; c < chunks; ++c, base += 64) {
+            if (partition_chunk(T, n, n / 2, bound, base, lane, res) != 0) {
+                state = 1;
+                break;
+            }
+        }
+    }
+    if (lane == 0) {
+        ServeOut o;
+        o.p = res;
+        o.k = base;
+        o.state = state;
+        o.pad = 0;
+        out[qi] = o;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1005,6 +1049,15 @@ hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t n
 
 hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32) {
     hipLaunchKernelGGL(prime_aux_kernel, dim3((n + 255) / 256), dim3(256), 0, s, sp, n, inv, c32);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_step(hipStream_t s, const Tables& T, const ServeIn* in, uint32_t count, uint64_t bound,
+                             uint32_t chunks, ServeOut* out) {
+    if (!count) return hipSuccess;
+    const uint32_t blocks = (count + SERVE_WAVES - 1) / SERVE_WAVES;
+    hipLaunchKernelGGL(serve_step_kernel, dim3(blocks), dim3(64 * SERVE_WAVES), 0, s, T, in, count, bound, chunks,
+                       out);
     return hipGetLastError();
 }
 

@@ -98,6 +98,21 @@ hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t n
                                  uint32_t* primes_out, uint32_t cap, uint32_t* count_out);
 hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32);
 
+// Serving step records: query n resumes its scan at candidate index k.
+struct ServeIn {
+    uint64_t n;
+    uint64_t k;
+};
+struct ServeOut {
+    uint64_t p;       // minimal p (state 1), 0 when none up to the bound
+    uint64_t k;       // next candidate index (state 0)
+    uint32_t state;   // 1 done, 0 continue
+    uint32_t pad;
+};
+
+hipError_t launch_serve_step(hipStream_t s, const Tables& T, const ServeIn* in, uint32_t count, uint64_t bound,
+                             uint32_t chunks, ServeOut* out);
+
 // Serving: p_out[i] = minimal p for ns[i].
 hipError_t launch_min_partition(hipStream_t s, const Tables& T, const uint64_t* ns, uint32_t count, uint64_t bound,
                                 uint64_t* p_out);

@@ -153,6 +153,64 @@ int gb_verify_windows(gb_ctx *ctx, const gb_window *windows, uint32_t count, uin
  * (single_check.cu:95-282), whose p is the first GPU finder, not the minimum. */
 int gb_min_partition_batch(gb_ctx *ctx, const uint64_t *n, uint32_t count, uint64_t *p_out);
 
+/* ---- serving loop with continuous batching ---------------------------- */
+
+/* A serving engine on one context: a host thread that keeps up to `slots`
+ * queries resident on the device and runs one step kernel after another
+ * over all of them.  Each step advances every resident query by up to
+ * `chunks_per_step` chunks of 64 candidate primes; queries that finish leave
+ * their slot at the step boundary and queued queries take the free slots for
+ * the next step, so arrivals join work already on the device instead of
+ * waiting for a batch to drain.  Every result is the minimal p, the same as
+ * gb_min_partition_batch gives for that n alone.
+ * Replaces single_check's one-number-per-process flow (single_check.cu:136-319). */
+typedef struct gb_server gb_server;
+
+typedef struct gb_serve_config {
+    uint32_t slots;            /* resident queries per step (0 = the context's serve_slots) */
+    uint32_t chunks_per_step;  /* 64-prime chunks per query per step (0 = 4) */
+    uint64_t mem_budget;       /* device bytes for slot state; slots shrink to fit (0 = no limit) */
+    uint64_t queue_cap;        /* queries waiting for a slot before submit says GB_EAGAIN (0 = 2^22) */
+} gb_serve_config;
+
+typedef struct gb_result {
+    uint64_t id;         /* submission number, from 0 in submit order */
+    uint64_t n;
+    uint64_t p;          /* smallest prime p with n-p prime, 0 if none up to phase2_limit */
+    uint64_t submit_ns;  /* CLOCK_MONOTONIC when submitted */
+    uint64_t done_ns;    /* CLOCK_MONOTONIC when its step completed */
+    uint32_t steps;      /* steps the query was resident for */
+    uint32_t reserved;
+} gb_result;
+
+typedef struct gb_serve_stats {
+    uint64_t submitted, completed;
+    uint64_t steps;         /* step kernels launched */
+    uint64_t step_queries;  /* sum over steps of resident queries */
+    uint64_t max_resident;
+    uint64_t queued;        /* waiting for a slot now */
+    uint64_t resident;      /* on the device now */
+    uint32_t slots;         /* slots after the memory budget */
+    uint32_t reserved;
+} gb_serve_stats;
+
+/* Starts the engine.  The context must stay open (gb_close refuses with
+ * GB_EAGAIN while any of its servers is open); other calls on the context
+ * may run while the server does. */
+int gb_serve_open(gb_ctx *ctx, const gb_serve_config *cfg, gb_server **out);
+/* Queues count even n >= 4 (all or none: GB_EINVAL for an odd n or n < 4,
+ * GB_EAGAIN when the queue has no room for all).  first_id (nullable) gets
+ * the id of n[0]; the others follow in order.  Thread-safe. */
+int gb_serve_submit(gb_server *srv, const uint64_t *n, uint32_t count, uint64_t *first_id);
+/* Moves up to max finished results into out, in completion order, waiting up
+ * to wait_ms (0: no wait) for the first.  *got = number moved; GB_EDEVICE if
+ * the engine stopped on a device error.  Thread-safe. */
+int gb_serve_poll(gb_server *srv, gb_result *out, uint32_t max, uint32_t *got, uint32_t wait_ms);
+int gb_serve_stats_get(gb_server *srv, gb_serve_stats *st);
+/* Stops the engine (queries not yet finished are dropped) and frees it. */
+int gb_serve_close(gb_server *srv);
+const char *gb_serve_last_error(const gb_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
