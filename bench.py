@@ -16,6 +16,8 @@ The JSON line carries:
   cpu_baseline  the CPU restatement (oracle/, OpenMP over segments, the
                 reference algorithm: per-n ascending prime scan) timed on this
                 host over 32 sampled 2^24-even segments spread over [4, 1e12].
+  high_windows  64 windows above 1e15 (q by pre-sieve + Miller-Rabin) in one
+                persistent launch, evens/s.
   serving       the continuous-batching server (gb_serve_*): p50/p99 latency
                 from scheduled arrival to result under open-loop Poisson
                 arrivals at 1e4, 1e5 and 1e6 queries/s, and queries/s for a
@@ -157,6 +159,30 @@ def serving(ctx) -> dict:
                     "[1e6, 2^64-2] (seed 1); headline p50/p99 at the lowest rate"}
 
 
+def high_windows(count: int = 64, lo_evens: int = 10**6, hi_evens: int = 10**9, seed: int = 0):
+    """64 windows of uneven length (uniform 1e6..1e9 evens) at even starts log-uniform in [1e15, 1.8e19]."""
+    rng = np.random.default_rng(seed)
+    starts = np.exp(rng.uniform(np.log(1e15), np.log(1.8e19), size=count))
+    lens = rng.integers(lo_evens, hi_evens + 1, size=count)
+    return [(int(a / 2) * 2, int(e)) for a, e in zip(starts, lens)]
+
+
+def windows_leg(ctx) -> dict:
+    """High windows: q decided by pre-sieve + Miller-Rabin, all windows in one persistent launch."""
+    wins = high_windows()
+    evens = sum(e for _, e in wins)
+    ctx.verify_windows([(a, min(e, 1 << 20)) for a, e in wins], P_SMALL)   # warm (tables, code)
+    device_sync()
+    t0 = time.perf_counter()
+    per, tot = ctx.verify_windows(wins, P_SMALL)
+    device_sync()
+    dt = time.perf_counter() - t0
+    assert tot["evens"] == evens and tot["failures"] == 0
+    return {"workload": "64 windows, uniform 1e6..1e9 evens, starts log-uniform in [1e15, 1.8e19] (seed 0)",
+            "evens": evens, "value": evens / dt, "unit": "evens/s", "seconds": dt,
+            "kernel_ms": tot["kernel_ms"], "phase2": tot["phase2"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +191,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-windows", action="store_true", help="skip the high-window (Miller-Rabin) leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -216,6 +243,7 @@ def main() -> int:
                  "kernel_ms": float(np.mean([o["kernel_ms"] for o in pouts])), "seg_evens": PLAIN_SEG}
 
     serve = None if args.no_serve else serving(ctx)
+    high = None if args.no_windows else windows_leg(ctx)
     ctx.close()
     cpu = None if args.no_cpu else cpu_baseline()
 
@@ -243,6 +271,7 @@ def main() -> int:
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "serving": serve,
+        "high_windows": high,
     }
     print(json.dumps(line))
     return 0

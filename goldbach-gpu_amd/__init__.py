@@ -26,6 +26,7 @@ BIN_DIR = os.path.join(PKG_DIR, "bin")
 
 GB_OK, GB_EINVAL, GB_ENOMEM, GB_EDEVICE, GB_ENODEV, GB_ERANGE, GB_EAGAIN = range(7)
 GB_WANT_PMIN = 0x1
+GB_TUNE_AUTO = 0x2
 GB_FORM_PLAIN = 0
 GB_FORM_BATCHED = 1
 
@@ -33,7 +34,7 @@ GB_FORM_BATCHED = 1
 EXPORTS = (
     "gb_open", "gb_close", "gb_last_error", "gb_device_count", "gb_device_info",
     "gb_sieve_window", "gb_verify_segment", "gb_verify_range", "gb_verify_windows",
-    "gb_min_partition_batch",
+    "gb_min_partition_batch", "gb_autotune",
     "gb_serve_open", "gb_serve_submit", "gb_serve_poll", "gb_serve_stats_get", "gb_serve_close",
     "gb_serve_last_error",
 )
@@ -113,6 +114,8 @@ def library() -> ctypes.CDLL:
         "gb_verify_windows": (ctypes.c_int, [vp, ctypes.POINTER(GbWindow), u32, u64, u32,
                                              ctypes.POINTER(GbSegStats), ctypes.POINTER(GbSegStats)]),
         "gb_min_partition_batch": (ctypes.c_int, [vp, vp, u32, vp]),
+        "gb_autotune": (ctypes.c_int, [vp, u64, u64, u64, u32, u32, ctypes.POINTER(GbTuning),
+                                       ctypes.POINTER(ctypes.c_double)]),
         "gb_serve_open": (ctypes.c_int, [vp, ctypes.POINTER(GbServeConfig), ctypes.POINTER(vp)]),
         "gb_serve_submit": (ctypes.c_int, [vp, vp, u32, ctypes.POINTER(u64)]),
         "gb_serve_poll": (ctypes.c_int, [vp, vp, u32, ctypes.POINTER(u32), u32]),
@@ -204,12 +207,24 @@ class Context:
 
     # -- whole range -------------------------------------------------------
     def verify_range(self, start: int, limit: int, p_small: int, form: int = GB_FORM_BATCHED,
-                     tile_log2: int = 0, seg_evens: int = 0, wg_per_cu: int = 0, want_pmin: bool = False) -> dict:
-        tun = GbTuning(form, tile_log2, seg_evens, wg_per_cu, GB_WANT_PMIN if want_pmin else 0)
+                     tile_log2: int = 0, seg_evens: int = 0, wg_per_cu: int = 0, want_pmin: bool = False,
+                     autotune: bool = False) -> dict:
+        flags = (GB_WANT_PMIN if want_pmin else 0) | (GB_TUNE_AUTO if autotune else 0)
+        tun = GbTuning(form, tile_log2, seg_evens, wg_per_cu, flags)
         st = GbSegStats()
         self._check(self._lib.gb_verify_range(self._h, start, limit, p_small, ctypes.byref(tun), ctypes.byref(st),
                                               None))
         return st.as_dict()
+
+    def autotune(self, start: int, limit: int, p_small: int, form: int = GB_FORM_BATCHED,
+                 want_pmin: bool = False) -> tuple:
+        """(tuning dict, evens/s on the samples; 0 when cached or the range is too short to sample)."""
+        t = GbTuning()
+        rate = ctypes.c_double(0)
+        self._check(self._lib.gb_autotune(self._h, start, limit, p_small, form, GB_WANT_PMIN if want_pmin else 0,
+                                          ctypes.byref(t), ctypes.byref(rate)))
+        return ({"form": t.form, "tile_log2": t.tile_log2, "seg_evens": t.seg_evens, "wg_per_cu": t.wg_per_cu,
+                 "flags": t.flags}, rate.value)
 
     def verify_windows(self, windows: Sequence[tuple], p_small: int, want_pmin: bool = False):
         arr = (GbWindow * len(windows))(*[GbWindow(int(a), int(e)) for a, e in windows])

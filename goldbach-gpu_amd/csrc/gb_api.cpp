@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -616,25 +617,155 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
     return GB_OK;
 }
 
+// Plain form over [start, limit]: one run_segment per seg evens, stopping at the first failure.
+static int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, const MarkParams& mp,
+                     gb_seg_stats* tot, volatile uint64_t* progress) {
+    uint64_t done = 0;
+    for (uint64_t a = start;;) {
+        const uint64_t left = (limit - a) / 2 + 1;
+        const uint64_t cnt = std::min(left, seg);
+        double ms = 0;
+        int rc = run_segment(c, a, cnt, mp, true, false, &ms);   // verified map written, as the reference does
+        if (rc) return rc;
+        gb_seg_stats one{};
+        to_stats(c->h_out[0], ms, &one);
+        add_stats(tot, one);
+        done += cnt;
+        if (progress) *progress = done;
+        if (one.failures) break;        // the reference stops at the first failure (goldbach.cu:420,476-480)
+        if (cnt == left) break;
+        a += 2 * cnt;
+    }
+    return GB_OK;
+}
+
+static int normalise_range(gb_ctx* c, uint64_t* start, uint64_t* limit) {
+    if (*limit & 1) --*limit;           // goldbach.cu:594
+    if (*start < 4) *start = 4;         // goldbach.cu:604-605
+    if (*start & 1) ++*start;
+    if (*limit < 4) return fail(c, GB_EINVAL, "LIMIT must be >= 4");
+    if (*start > *limit) return fail(c, GB_EINVAL, "START must not exceed LIMIT");
+    return GB_OK;
+}
+
+int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uint32_t form, uint32_t flags,
+                gb_tuning* best, double* evens_per_s) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (!best) return fail(c, GB_EINVAL, "gb_autotune: best is NULL");
+    int rc = normalise_range(c, &start, &limit);
+    if (rc) return rc;
+    if (form != GB_FORM_PLAIN && form != GB_FORM_BATCHED) return fail(c, GB_EINVAL, "unknown form");
+    flags &= GB_WANT_PMIN;
+    const std::vector<uint64_t> key = {form, start, limit, p_small, flags};
+    for (auto& e : c->tune_cache)
+        if (e.first == key) {
+            *best = e.second;
+            if (evens_per_s) *evens_per_s = 0;
+            return GB_OK;
+        }
+    HCK(c, hipSetDevice(c->device));
+    gb_tuning def{};
+    def.form = form;
+    def.flags = flags;
+    def.tile_log2 = DEFAULT_TILE_LOG2;
+    def.wg_per_cu = 1;
+    def.seg_evens = std::min<uint64_t>(DEFAULT_SEG, c->cfg.max_seg_evens);
+    const uint64_t evens = (limit - start) / 2 + 1;
+    // four sample windows at 1/8, 3/8, 5/8 and 7/8 of the range: the sieve's cost grows with q
+    const uint64_t win = std::min<uint64_t>(evens / 4, form == GB_FORM_BATCHED ? (1ull << 30) : (1ull << 29));
+    if (win < (1ull << 22)) {
+        *best = def;
+        if (evens_per_s) *evens_per_s = 0;
+        return GB_OK;
+    }
+    std::vector<WinDesc> sample;
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t mid = (evens / 8) * (2 * k + 1);
+        const uint64_t first = std::min(mid - std::min(mid, win / 2), evens - win);
+        sample.push_back(WinDesc{start + 2 * first, win, 0, 0, 0, 0, 0, 0, 0});
+    }
+    std::vector<gb_tuning> cand;
+    if (form == GB_FORM_BATCHED) {
+        for (uint32_t tl = 16; tl <= 20; ++tl) {
+            ItemTable it;
+            rc = build_items(c, (1u << tl) + HALO + WIN_PAD, &it);
+            if (rc) return rc;
+            const size_t lds = fused_lds_bytes(1u << tl, it.n_med);
+            for (uint32_t wg = 1; wg <= 2; ++wg) {
+                if (lds * wg > LDS_PER_CU) continue;
+                gb_tuning t = def;
+                t.tile_log2 = tl;
+                t.wg_per_cu = wg;
+                cand.push_back(t);
+            }
+        }
+    } else {
+        for (uint32_t sl = 24; sl <= 29; ++sl) {
+            if ((1ull << sl) > c->cfg.max_seg_evens) break;
+            gb_tuning t = def;
+            t.seg_evens = 1ull << sl;
+            cand.push_back(t);
+        }
+        if (cand.empty()) cand.push_back(def);
+    }
+    const MarkParams mp = mark_params(c, p_small, flags);
+    double best_s = 0;
+    gb_tuning pick = def;
+    for (const gb_tuning& t : cand) {
+        double secs = 1e30;
+        for (int rep = 0; rep < 3; ++rep) {   // the first run of a candidate builds its schedule and graph
+            const auto t0 = std::chrono::steady_clock::now();
+            if (form == GB_FORM_BATCHED) {
+                double ms = 0;
+                rc = run_batched(c, sample, p_small, flags, t.tile_log2, t.wg_per_cu, nullptr, &ms);
+                if (rc) return rc;
+            } else {
+                for (const WinDesc& w : sample) {
+                    gb_seg_stats tot{};
+                    rc = run_plain(c, w.start, w.start + 2 * (w.evens - 1), t.seg_evens, mp, &tot, nullptr);
+                    if (rc) return rc;
+                }
+            }
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (rep > 0) secs = std::min(secs, dt);
+        }
+        const double rate = 4.0 * (double)win / secs;
+        if (rate > best_s) {
+            best_s = rate;
+            pick = t;
+        }
+    }
+    c->tune_cache.emplace_back(key, pick);
+    *best = pick;
+    if (evens_per_s) *evens_per_s = best_s;
+    return GB_OK;
+}
+
 int gb_verify_range(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, const gb_tuning* tun,
                     gb_seg_stats* st, volatile uint64_t* progress) {
     if (!c) return GB_EINVAL;
     c->err.clear();
-    if (limit & 1) limit--;             // goldbach.cu:594
-    if (start < 4) start = 4;           // goldbach.cu:604-605
-    if (start & 1) start++;
-    if (limit < 4) return fail(c, GB_EINVAL, "LIMIT must be >= 4");
-    if (start > limit) return fail(c, GB_EINVAL, "START must not exceed LIMIT");
+    int rc0 = normalise_range(c, &start, &limit);
+    if (rc0) return rc0;
     HCK(c, hipSetDevice(c->device));
     gb_tuning t{};
     if (tun) t = *tun;
+    if (t.flags & GB_TUNE_AUTO) {
+        gb_tuning a{};
+        rc0 = gb_autotune(c, start, limit, p_small, t.form, t.flags, &a, nullptr);
+        if (rc0) return rc0;
+        t.tile_log2 = a.tile_log2;
+        t.wg_per_cu = a.wg_per_cu;
+        t.seg_evens = a.seg_evens;
+    }
     const uint64_t evens = (limit - start) / 2 + 1;
     gb_seg_stats tot{};
     if (t.form == GB_FORM_BATCHED) {
         std::vector<WinDesc> w(1);
         w[0] = WinDesc{start, evens, 0, 0, 0, 0, 0, 0, 0};
         double ms = 0;
-        int rc = run_batched(c, w, p_small, t.flags, t.tile_log2, t.wg_per_cu, nullptr, &ms);
+        int rc = run_batched(c, w, p_small, t.flags & GB_WANT_PMIN, t.tile_log2, t.wg_per_cu, nullptr, &ms);
         if (rc) return rc;
         to_stats(c->h_out[0], ms, &tot);
         if (progress) *progress = evens;
@@ -645,22 +776,8 @@ int gb_verify_range(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small,
     uint64_t seg = t.seg_evens ? t.seg_evens : std::min<uint64_t>(DEFAULT_SEG, c->cfg.max_seg_evens);
     if (seg > c->cfg.max_seg_evens) return fail(c, GB_ERANGE, "seg_evens above max_seg_evens");
     const MarkParams mp = mark_params(c, p_small, t.flags);
-    uint64_t done = 0;
-    for (uint64_t a = start;;) {
-        const uint64_t left = (limit - a) / 2 + 1;
-        const uint64_t cnt = std::min(left, seg);
-        double ms = 0;
-        int rc = run_segment(c, a, cnt, mp, true, false, &ms);   // verified map written, as the reference does
-        if (rc) return rc;
-        gb_seg_stats one{};
-        to_stats(c->h_out[0], ms, &one);
-        add_stats(&tot, one);
-        done += cnt;
-        if (progress) *progress = done;
-        if (one.failures) break;        // the reference stops at the first failure (goldbach.cu:420,476-480)
-        if (cnt == left) break;
-        a += 2 * cnt;
-    }
+    int rc = run_plain(c, start, limit, seg, mp, &tot, progress);
+    if (rc) return rc;
     if (st) *st = tot;
     return GB_OK;
 }

@@ -7,6 +7,7 @@
 
 #include <atomic>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gb_kernels.hpp"
@@ -77,6 +78,8 @@ struct gb_ctx {
     // cached batched graph
     hipGraphExec_t graph = nullptr;
     std::vector<uint64_t> graph_key;
+    // autotuner results: key (form, start, limit, p_small, flags) -> tuning
+    std::vector<std::pair<std::vector<uint64_t>, gb_tuning>> tune_cache;
     // serving engines running on this context's tables
     std::atomic<int> servers{0};
 };

@@ -1000,12 +1000,15 @@ hipError_t launch_residues(hipStream_t s, const Tables& T, const WinDesc* wins, 
     return hipGetLastError();
 }
 
+size_t fused_lds_bytes(uint32_t tile_evens, uint32_t n_med) {
+    return (size_t)((tile_evens + HALO + WIN_PAD) / 32 + 2) * 4 +
+           (size_t)(((n_med > NPAT ? n_med - NPAT : 1) + 1) & ~1u) * 4 + sizeof(LaneStats) * (FUSED_BS / 64) + 8;
+}
+
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
                         uint64_t* verified, SlowList sl, StatRec* rec, const uint32_t* r0) {
-    const size_t lds = (size_t)((tile_evens + HALO + WIN_PAD) / 32 + 2) * 4 +
-                       (size_t)(((T.n_med > NPAT ? T.n_med - NPAT : 1) + 1) & ~1u) * 4 +
-                       sizeof(LaneStats) * (FUSED_BS / 64) + 8;
+    const size_t lds = fused_lds_bytes(tile_evens, T.n_med);
     hipError_t e = allow_lds(fused_kernel<FUSED_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fused_kernel<FUSED_BS>, dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin, ntiles,

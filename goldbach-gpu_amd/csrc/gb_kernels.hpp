@@ -75,6 +75,10 @@ hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams&
                                uint64_t* verified, uint32_t* pmin_out, SlowList sl, StatRec* rec,
                                uint32_t* nrec_out);
 
+// LDS bytes of one fused workgroup for tiles of tile_evens (n_med: Tables::n_med of its schedule).
+size_t fused_lds_bytes(uint32_t tile_evens, uint32_t n_med);
+constexpr size_t LDS_PER_CU = 160 * 1024;
+
 // Fused persistent kernel over the tiles of `nwin` windows.
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,

@@ -8,7 +8,8 @@
 // Additions: --form=batched|plain (default batched: one persistent fused
 // launch; plain: one launch sequence and host sync per segment, as the
 // reference), --tile-log2=N (batched tile of 2^N evens), --phase2-limit=N
-// (Phase-2 prime bound, 1e8 as in the reference; a test hook).
+// (Phase-2 prime bound, 1e8 as in the reference; a test hook), --autotune
+// (time tile / segment-size candidates on samples of the range first).
 // --batch-size and --async are accepted and have no effect: the device keeps
 // every candidate prime resident, so there are no prime batches to size.
 #include <atomic>
@@ -42,6 +43,7 @@ void print_usage(const char* prog) {
               << "  --form=F         batched (default) or plain per-segment launches\n"
               << "  --tile-log2=N    Batched tile of 2^N evens (12..20, default 20)\n"
               << "  --phase2-limit=N Phase-2 prime bound (default: 100000000)\n"
+              << "  --autotune       Pick tile / segment size by timing samples of the range\n"
               << "  -h, --help       Show this help message\n";
 }
 
@@ -61,6 +63,7 @@ int main(int argc, char** argv) {
     uint32_t tile_log2 = 0;
     int requested_gpus = 1;
     bool progress = false;
+    bool autotune = false;
     uint32_t form = GB_FORM_BATCHED;
     std::vector<std::string> positional;
 
@@ -74,6 +77,7 @@ int main(int argc, char** argv) {
             }
             if (a == "--async") continue;
             if (a == "--progress") { progress = true; continue; }
+            if (a == "--autotune") { autotune = true; continue; }
             if (has("--batch-size=")) { (void)std::stoull(a.substr(13)); continue; }
             if (has("--gpus=")) { requested_gpus = std::stoi(a.substr(7)); continue; }
             if (has("--seg-size=")) { SEG_SIZE = std::stoull(a.substr(11)); continue; }
@@ -207,6 +211,21 @@ int main(int argc, char** argv) {
     tun.form = form;
     tun.seg_evens = SEG_SIZE;
     tun.tile_log2 = tile_log2;
+    if (autotune) {
+        double rate = 0;
+        if (gb_autotune(ctx, START, LIMIT, P_SMALL, form, 0, &tun, &rate) != GB_OK) {
+            std::cerr << "[!] FATAL ERROR in GPU 0 Worker: " << gb_last_error(ctx) << "\n";
+            gb_close(ctx);
+            return 1;
+        }
+        if (form == GB_FORM_BATCHED)
+            std::cout << "Autotuned      : tile 2^" << tun.tile_log2 << " evens, " << tun.wg_per_cu
+                      << " workgroup(s) per CU";
+        else
+            std::cout << "Autotuned      : segments of " << tun.seg_evens << " evens";
+        std::cout << " (" << std::scientific << std::setprecision(3) << rate << std::defaultfloat
+                  << " evens/s on the samples)\n\n";
+    }
     gb_seg_stats st{};
     const int rc = gb_verify_range(ctx, START, LIMIT, P_SMALL, &tun, &st, &processed);
     const double total_ms = secs_since(t_main) * 1000.0;

@@ -118,11 +118,25 @@ int gb_verify_segment(gb_ctx *ctx, uint64_t seg_start, uint64_t seg_even_count, 
 
 typedef struct gb_tuning {
     uint32_t form;          /* GB_FORM_* */
-    uint32_t tile_log2;     /* batched: evens per tile = 2^tile_log2 (0 = autotuned/default) */
+    uint32_t tile_log2;     /* batched: evens per tile = 2^tile_log2 (0 = 20) */
     uint64_t seg_evens;     /* plain: evens per segment (0 = 2^27) */
     uint32_t wg_per_cu;     /* batched: persistent workgroups per CU (0 = default) */
-    uint32_t flags;         /* GB_WANT_PMIN */
+    uint32_t flags;         /* GB_WANT_PMIN | GB_TUNE_AUTO */
 } gb_tuning;
+
+#define GB_TUNE_AUTO 0x2u   /* gb_tuning.flags: take tile / segment size from gb_autotune (cached per context) */
+
+/* Times candidate tunings of `form` on sample windows spread over
+ * [start, limit] and writes the fastest to *best (its form and flags as
+ * given); *evens_per_s (nullable) gets the sample rate it reached.
+ *   batched: tile_log2 in 16..20 x wg_per_cu in {1, 2}, where the LDS fits a CU
+ *   plain:   seg_evens in 2^24..2^29 (up to max_seg_evens)
+ * Ranges too short to sample return the defaults.  The result is cached on
+ * the context per (form, start, limit, p_small, flags).  Generalises the
+ * reference's offline grid over SEG_SIZE / P_SMALL / BATCH_SIZE
+ * (benchmarks/bench_5.sh:10-13) to an on-device search. */
+int gb_autotune(gb_ctx *ctx, uint64_t start, uint64_t limit, uint64_t p_small, uint32_t form, uint32_t flags,
+                gb_tuning *best, double *evens_per_s);
 
 /* All even n in [start, limit] (start even >= 4 after the reference's
  * adjustments, goldbach.cu:594,604-609).  Replaces main's worker spawn and

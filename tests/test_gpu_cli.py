@@ -24,6 +24,12 @@ def test_basic_run(form):
     assert rc == 0 and "satisfy Goldbach. ✓" in out and "Phase 2 fallbacks      : 0" in out
 
 
+@pytest.mark.parametrize("form", ["batched", "plain"])
+def test_autotune_flag(form):
+    rc, out = run([GOLDBACH, "100000000000", "--seg-size=134217728", f"--form={form}", "--autotune"])
+    assert rc == 0 and "Autotuned      :" in out and "satisfy Goldbach. ✓" in out
+
+
 def test_progress_flag():
     rc, out = run([GOLDBACH, "20000000000", "--seg-size=10000000", "--p-small=1000000", "--batch-size=1000000",
                    "--progress"])

@@ -142,6 +142,46 @@ def test_windows_match_oracle(ctx, orc):
     assert tot["evens"] == evens
 
 
+def test_autotune_picks_a_valid_tuning(ctx, gb):
+    want = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_PLAIN, seg_evens=1 << 27)
+    t, rate = ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED)
+    assert 16 <= t["tile_log2"] <= 20 and t["wg_per_cu"] in (1, 2) and rate > 1e10
+    assert ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED)[0] == t   # cached
+    got = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED, autotune=True)
+    assert (got["evens"], got["phase2"], got["failures"]) == (want["evens"], want["phase2"], want["failures"])
+    tp, rate_p = ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_PLAIN)
+    assert tp["seg_evens"] in [1 << k for k in range(24, 28)] and rate_p > 1e9
+    got = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_PLAIN, autotune=True)
+    assert (got["evens"], got["phase2"], got["failures"]) == (want["evens"], want["phase2"], want["failures"])
+    # a range too short to sample keeps the defaults
+    short, r0 = ctx.autotune(4, 10**6, 10**6)
+    assert (short["tile_log2"], short["wg_per_cu"], r0) == (20, 1, 0)
+
+
+def high_windows(count, lo_evens, hi_evens, seed=0):
+    """Windows of uneven length starting at even offsets log-uniform in [1e15, 1.8e19]."""
+    rng = np.random.default_rng(seed)
+    starts = np.exp(rng.uniform(np.log(1e15), np.log(1.8e19), size=count))
+    lens = rng.integers(lo_evens, hi_evens + 1, size=count)
+    return [(int(a / 2) * 2, int(e)) for a, e in zip(starts, lens)]
+
+
+def test_high_windows_draw_matches_oracle(ctx, orc):
+    # the high-window workload's draw (seed 0), lengths scaled down so the oracle finishes in seconds
+    wins = high_windows(64, 1000, 30_000)
+    per, tot = ctx.verify_windows(wins, 10**6, want_pmin=True)
+    for (a, e), got in zip(wins, per):
+        st, _, _ = orc.verify_segment(a, e, 10**6, want_bits=False, want_pmin=False)
+        for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
+            assert got[k] == st[k], (a, e, k)
+    assert tot["evens"] == sum(e for _, e in wins) and tot["failures"] == 0
+    # the same windows one by one through the plain per-segment path
+    for (a, e), got in zip(wins[:8], per[:8]):
+        r = ctx.verify_segment(a, e, 10**6, want_pmin=True)
+        for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
+            assert r.stats[k] == got[k], (a, k)
+
+
 def test_min_partition_matches_oracle(ctx, orc):
     rng = np.random.default_rng(1)
     n = (np.exp(rng.uniform(np.log(1e6), np.log(2.0**64 - 2), size=3000)) / 2).astype(np.uint64) * np.uint64(2)
