@@ -172,13 +172,18 @@ int ensure_r0(gb_ctx* c, uint64_t n) {
     return GB_OK;
 }
 
-// Residue bookkeeping for a window whose first sieved q is rbase and whose
-// largest q is q_top: every sieving prime any of its tiles may use.
+// Sieving-prime entries for a window whose first sieved q is rbase and whose
+// largest q is q_top: every prime any of its tiles may use.  A tile whose
+// sqrt(q) passes the table (2^22) only pre-sieves to MR_PRESIEVE and decides
+// the survivors by Miller-Rabin.
 void set_residues(const gb_ctx* c, WinDesc* wd, uint64_t rbase, uint64_t q_top, uint64_t r0off) {
-    const uint64_t s = sieve_prime_bound(q_top);
+    const uint64_t s_top = sieve_prime_bound(q_top);
+    const uint64_t s_low = sieve_prime_bound(rbase);
+    uint64_t bound = s_top;
+    if (s_top > SP_MAX) bound = s_low > SP_MAX ? (uint64_t)MR_PRESIEVE : (uint64_t)SP_MAX;
     wd->rbase = rbase;
     wd->r0off = r0off;
-    wd->np = count_le(c->h_sp, std::min<uint64_t>(s, SP_MAX));
+    wd->np = count_le(c->h_sp, bound);
     wd->pad = 0;
     wd->pad2 = 0;
 }
@@ -248,7 +253,15 @@ int build_tables(gb_ctx* c) {
     const uint64_t sp_words = (sp_bits + 63) / 64;
     uint64_t* d_spbits = nullptr;
     HCK(c, dalloc(&d_spbits, sp_words));
-    HCK(c, launch_sieve_window(s, seedT, 3, 0, sp_bits, PLAIN_TILE_BITS, d_spbits, 0, nullptr));
+    // the seed primes as the window [3, 2^22) sees them
+    WinDesc* d_w = nullptr;
+    PrimeRes* d_pr = nullptr;
+    HCK(c, dalloc(&d_w, 1));
+    HCK(c, dalloc(&d_pr, nseed));
+    const WinDesc w_seed{3, 0, 0, 0, 3, 0, nseed, 0, 0};
+    HCK(c, hipMemcpyAsync(d_w, &w_seed, sizeof w_seed, hipMemcpyHostToDevice, s));
+    HCK(c, launch_residues(s, seedT, d_w, 1, nseed, d_pr));
+    HCK(c, launch_sieve_window(s, seedT, 3, 0, sp_bits, PLAIN_TILE_BITS, d_spbits, 0, d_pr));
     HCK(c, dalloc(&c->d_sp, SP_TABLE_CAP));
     HCK(c, launch_compact_primes(s, d_spbits, (uint32_t)sp_words, SP_MAX - 1, c->d_sp, SP_TABLE_CAP, d_cnt));
     HCK(c, hipMemcpyAsync(&c->nsp, d_cnt, 4, hipMemcpyDeviceToHost, s));
@@ -280,8 +293,18 @@ int build_tables(gb_ctx* c) {
     Tables T = make_tables(c, it);
     T.p2bits = nullptr;
     T.p2_limit = 0;
-    HCK(c, launch_sieve_window(s, T, 3, 0, p2_bits, PLAIN_TILE_BITS, c->d_p2bits, 0, nullptr));
+    WinDesc w_p2{3, 0, 0, 0, 0, 0, 0, 0, 0};
+    set_residues(c, &w_p2, 3, lim, 0);
     HCK(c, hipStreamSynchronize(s));
+    dfree(d_pr);
+    d_pr = nullptr;
+    HCK(c, dalloc(&d_pr, w_p2.np));
+    HCK(c, hipMemcpyAsync(d_w, &w_p2, sizeof w_p2, hipMemcpyHostToDevice, s));
+    HCK(c, launch_residues(s, T, d_w, 1, w_p2.np, d_pr));
+    HCK(c, launch_sieve_window(s, T, 3, 0, p2_bits, PLAIN_TILE_BITS, c->d_p2bits, 0, d_pr));
+    HCK(c, hipStreamSynchronize(s));
+    dfree(d_w);
+    dfree(d_pr);
     return GB_OK;
 }
 

@@ -65,7 +65,7 @@ struct gb_ctx {
     uint64_t* d_rec_hi = nullptr;
     uint32_t wins_cap = 0;
     uint64_t* d_counter = nullptr;
-    uint32_t* d_r0 = nullptr;          // per-window residues rbase mod sp[k]
+    gbk::PrimeRes* d_r0 = nullptr;     // per-window sieving-prime entries
     uint64_t r0_cap = 0;
     // serving buffers
     uint64_t* d_qn = nullptr;

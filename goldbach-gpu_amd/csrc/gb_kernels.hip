@@ -50,24 +50,64 @@ __device__ __forceinline__ uint32_t count_le(const uint32_t* sp, uint32_t n, uin
 
 __device__ __noinline__ bool is_prime_u64_call(uint64_t q) { return is_prime_u64(q); }
 
-// Clear bit i of the LDS sieve window.  One LDS instruction per mark: a CU
+// Clearing bits of the LDS sieve window.  One LDS instruction per mark: a CU
 // retires about one wave64 LDS instruction per 4.5 clocks (ds_and_b32 and
-// plain stores alike; ds_mskor_b32 is slower), which with ~5 VALU per mark
-// keeps the two units roughly balanced.
-__device__ __forceinline__ void lds_clear_bit(uint32_t* win32, uint32_t i) {
-    atomicAnd(&win32[i >> 5], ~(1u << (i & 31)));
+// plain stores alike; ds_mskor_b32 is slower).  Along a run i, i+p, i+2p, ...
+// the clear-mask ~(1 << (i & 31)) rotates left by p mod 32 per step, so a
+// mark costs one v_alignbit_b32 for the mask, two ops for the word address
+// and the index add (the window sits at a fixed LDS address).
+
+// ~(1u << (i & 31)) as a rotation of ~1.
+__device__ __forceinline__ uint32_t clear_mask(uint32_t i) { return __builtin_amdgcn_alignbit(~1u, ~1u, 0u - i); }
+
+__device__ __forceinline__ void lds_clear_bit(uint32_t* win32, uint32_t i) { atomicAnd(&win32[i >> 5], clear_mask(i)); }
+
+// bits i, i + p, ... below end
+__device__ __forceinline__ void clear_run(uint32_t* win32, uint32_t i, uint32_t p, uint32_t end) {
+    uint32_t mask = clear_mask(i);
+    const uint32_t rot = 0u - p;   // rotate left by p mod 32
+    for (; i < end; i += p) {
+        atomicAnd(&win32[i >> 5], mask);
+        mask = __builtin_amdgcn_alignbit(mask, mask, rot);
+    }
+}
+
+// exactly N bits i, i + p, ..., i + (N-1)p
+template <int N>
+__device__ __forceinline__ void clear_run_n(uint32_t* win32, uint32_t i, uint32_t p) {
+    uint32_t mask = clear_mask(i);
+    const uint32_t rot = 0u - p;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+        atomicAnd(&win32[i >> 5], mask);
+        mask = __builtin_amdgcn_alignbit(mask, mask, rot);
+        i += p;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // sieve_tile: window bits [g0, g0 + nbits) of a window whose bit i (>= lead)
 // holds q = qv0 + 2(i - lead).  Result in win32[0 .. ceil(nbits/32)).
-// r0[k] = rbase mod sp[k] (rbase <= q_lo, q_lo - rbase < 2^52) for the first
-// sieving primes; without it the residues come from two fp64 reductions.
+// pr[k] describes sieving prime sp[k] relative to the odd rbase <= q_lo
+// (q_lo - rbase < 2^52): its first multiple sits e bits above rbase, so in
+// this tile at bit lb + ((e - (q_lo - rbase)/2) mod p), one fp64 reduction.
 // ---------------------------------------------------------------------------
+
+// (e - db) mod p for y = e - db exact in fp64 (|y| < 2^52): the quotient
+// estimate is off by at most one, so the remainder lies in (-p, 2p) and two
+// unsigned min fix-ups finish it.
+__device__ __forceinline__ uint32_t first_off(double y, uint32_t p, double inv) {
+    const double q = floor(y * inv);
+    uint32_t u = (uint32_t)(int32_t)fma(-q, (double)p, y);
+    u = min(u, u + p);   // negative: u wrapped high, u + p is the residue
+    u = min(u, u - p);   // >= p: u - p is the residue; else u - p wraps high
+    return u;
+}
+
 template <int BS>
 __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ med_i0, uint64_t qv0,
                            uint32_t lead, uint64_t g0, uint32_t nbits, const Tables& T, uint32_t tid,
-                           const uint32_t* __restrict__ r0, uint64_t rbase) {
+                           const PrimeRes* __restrict__ pr, uint64_t rbase) {
     uint64_t* win64 = reinterpret_cast<uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
     const uint64_t b_lo = g0 > lead ? g0 : (uint64_t)lead;
@@ -78,7 +118,7 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     const uint64_t s = isqrt64(q_top);
     const bool mr = s > T.cover;
     const uint32_t np = count_le(T.sp, T.nsp, mr ? (uint64_t)MR_PRESIEVE : s);
-    const uint64_t dq = q_lo - rbase;   // < 2^52 whenever r0 is given
+    const double dbf = (double)((q_lo - rbase) >> 1);   // bits from rbase to q_lo, < 2^51
 
     // 1) primes 3..61 as per-word masks: bit j of word k is q(64k+j); the
     //    multiples of p in a word sit at j0, j0+p, ... with j0 stepping down by
@@ -99,8 +139,8 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
         for (int i = 0; i < NPAT; ++i) {
             const uint32_t p = pat_p(i);
             acc |= pat_mask(p) << j0[i];
-            const int t = (int)j0[i] - (int)((64u * BS) % p);
-            j0[i] = t < 0 ? (uint32_t)(t + (int)p) : (uint32_t)t;
+            const uint32_t t = j0[i] - (64u * BS) % p;   // wraps high when negative
+            j0[i] = min(t, t + p);
         }
         uint64_t w = ~acc;
         if (low) {
@@ -116,11 +156,12 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     // 2) first marks of the medium primes (split into CHUNK-mark items)
     const uint32_t kmed_end = np < T.n_med ? np : T.n_med;
     for (uint32_t k = NPAT + tid; k < kmed_end; k += BS) {
-        const uint32_t p = T.sp[k];
         uint64_t i0 = nbits;
         if (any_valid) {
-            const uint32_t r = r0 ? mod_from_base(dq, r0[k], p, T.inv[k]) : mod_small(q_lo, p, T.inv[k], T.c32[k]);
-            i0 = first_mark_r(q_lo, r, lb, p);
+            const PrimeRes e = pr[k];
+            i0 = lb + first_off((double)e.e - dbf, e.p, e.inv);
+            const uint64_t pp = (uint64_t)e.p * e.p;   // the first mark is p^2 or above
+            if (q_lo < pp) i0 = max(i0, lb + ((pp - q_lo) >> 1));
         }
         med_i0[k - NPAT] = i0 < nbits ? (uint32_t)i0 : nbits;
     }
@@ -151,48 +192,34 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
                 p_n = T.items_p[un];
             }
             const uint32_t i = med_i0[(it >> 12) - NPAT] + (it & 4095u) * (CHUNK * p);
-            if (i + (CHUNK - 1) * p < nbits) {
-#pragma unroll
-                for (uint32_t m = 0; m < CHUNK; ++m) lds_clear_bit(win32, i + m * p);
-            } else {
-                for (uint32_t m = i; m < nbits; m += p) lds_clear_bit(win32, m);
-            }
+            if (i + (CHUNK - 1) * p < nbits)
+                clear_run_n<CHUNK>(win32, i, p);
+            else
+                clear_run(win32, i, p, nbits);
             u = un;
         }
     }
-    // 3b) large primes, one per thread; the next prime's table entries are
-    //     loaded before this prime's marks issue.  Primes p <= sqrt(q_lo) need
-    //     no p^2 adjustment (all but the last few of a tile).
+    // 3b) large primes, one per thread; the next prime's entry is loaded
+    //     before this prime's marks issue.  Primes p <= sqrt(q_lo) need no
+    //     p^2 adjustment (all but the last few of a tile).
     if (any_valid) {
-        const double dqf = (double)dq;
         const uint64_t root_lo = isqrt64(q_lo);
-        uint32_t k = T.n_med + tid, p_n = 0, r_n = 0;
-        double inv_n = 0.0;
-        if (k < np) {
-            p_n = T.sp[k];
-            inv_n = T.inv[k];
-            r_n = r0 ? r0[k] : T.c32[k];
-        }
+        uint32_t k = T.n_med + tid;
+        PrimeRes nx{0, 0, 0.0};
+        if (k < np) nx = pr[k];
         while (k < np) {
-            const uint32_t p = p_n, rr = r_n, kn = k + BS;
-            const double iv = inv_n;
-            if (kn < np) {
-                p_n = T.sp[kn];
-                inv_n = T.inv[kn];
-                r_n = r0 ? r0[kn] : T.c32[kn];
+            const PrimeRes e = nx;
+            const uint32_t kn = k + BS;
+            if (kn < np) nx = pr[kn];
+            uint32_t i0 = lb + first_off((double)e.e - dbf, e.p, e.inv);
+            if (e.p > root_lo) {
+                const uint64_t pp = (uint64_t)e.p * e.p;
+                if (q_lo < pp) {
+                    const uint64_t f = lb + ((pp - q_lo) >> 1);
+                    i0 = f < nbits ? max(i0, (uint32_t)f) : nbits;
+                }
             }
-            const uint32_t r = r0 ? mod_base_u32(dqf, rr, p, iv) : mod_small(q_lo, p, iv, rr);
-            uint32_t i0;
-            if (p <= root_lo) {
-                uint32_t t = p - r;
-                t = t == p ? 0u : t;
-                t += (t & 1u) * p;       // the odd multiple
-                i0 = lb + (t >> 1);
-            } else {
-                const uint64_t f = first_mark_r(q_lo, r, lb, p);
-                i0 = f < nbits ? (uint32_t)f : nbits;
-            }
-            for (uint32_t m = i0; m < nbits; m += p) lds_clear_bit(win32, m);
+            clear_run(win32, i0, e.p, nbits);
             k = kn;
         }
     }
@@ -309,10 +336,21 @@ constexpr MarkTable make_mark_table() {
 constexpr MarkTable kMark = make_mark_table();
 static_assert(kMark.p[0] == 3 && kMark.p[NMARK - 1] == 8191, "odd primes <= 8193");
 
+// Bits [S, S+64) of hi:lo with S a compile-time constant: two v_alignbit_b32
+// (a 64-bit shift pair would cost three instructions).
 template <uint32_t S>
 __device__ __forceinline__ uint64_t funnel_c(uint64_t lo, uint64_t hi) {
-    if constexpr (S == 0) return lo;
-    else return (lo >> S) | (hi << (64 - S));
+    const uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32);
+    const uint32_t h0 = (uint32_t)hi, h1 = (uint32_t)(hi >> 32);
+    if constexpr (S == 0) {
+        return lo;
+    } else if constexpr (S < 32) {
+        return ((uint64_t)__builtin_amdgcn_alignbit(h0, l1, S) << 32) | __builtin_amdgcn_alignbit(l1, l0, S);
+    } else if constexpr (S == 32) {
+        return ((uint64_t)h0 << 32) | l1;
+    } else {
+        return ((uint64_t)__builtin_amdgcn_alignbit(h1, h0, S - 32) << 32) | __builtin_amdgcn_alignbit(h0, l1, S - 32);
+    }
 }
 
 template <int K>
@@ -563,15 +601,15 @@ __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first,
 template <int BS>
 __global__ __launch_bounds__(BS) void sieve_window_kernel(Tables T, uint64_t qv0, uint32_t lead, uint64_t nbits,
                                                           uint32_t tile_bits, uint64_t* __restrict__ out,
-                                                          int tail_ones, const uint32_t* __restrict__ r0) {
-    extern __shared__ uint32_t lds[];
-    uint32_t* win32 = lds;
-    uint32_t* med_i0 = lds + tile_bits / 32 + 2;
+                                                          int tail_ones, const PrimeRes* __restrict__ pr) {
+    __shared__ __align__(16) uint32_t win32[SIEVE_WIN_WORDS];   // fixed LDS address: folds into ds offsets
+    extern __shared__ uint32_t lds[];                             // medium-prime first marks
+    uint32_t* med_i0 = lds;
     const uint32_t tid = threadIdx.x;
     const uint64_t g0 = (uint64_t)blockIdx.x * tile_bits;
     if (g0 >= nbits) return;
     const uint32_t tb = (uint32_t)(nbits - g0 < tile_bits ? nbits - g0 : tile_bits);
-    sieve_tile<BS>(win32, med_i0, qv0, lead, g0, tb, T, tid, r0, qv0);
+    sieve_tile<BS>(win32, med_i0, qv0, lead, g0, tb, T, tid, pr, qv0);
     const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
     const uint32_t nw = (tb + 63) >> 6;
     const uint64_t wo = g0 >> 6;
@@ -624,14 +662,14 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
                                                    uint32_t nwin, uint64_t ntiles, uint32_t tile_evens,
                                                    uint64_t* __restrict__ tile_counter,
                                                    uint64_t* __restrict__ verified, SlowList sl,
-                                                   StatRec* __restrict__ rec, const uint32_t* __restrict__ r0) {
+                                                   StatRec* __restrict__ rec, const PrimeRes* __restrict__ pr) {
+    // the window at a fixed LDS address (sized for the largest tile); dynamic
+    // LDS after it: [medium first marks | stats | tile id]
+    __shared__ __align__(16) uint32_t win32[FUSED_WIN_WORDS];
     extern __shared__ uint32_t lds[];
-    // dynamic LDS only (window at offset 0): [window | medium first marks | stats | tile id]
-    const uint32_t win_words32 = (tile_evens + HALO + WIN_PAD) / 32 + 2;
     const uint32_t med_words = ((T.n_med > NPAT ? T.n_med - NPAT : 1) + 1) & ~1u;
-    uint32_t* win32 = lds;
-    uint32_t* med_i0 = lds + win_words32;
-    LaneStats* red = reinterpret_cast<LaneStats*>(lds + win_words32 + med_words);
+    uint32_t* med_i0 = lds;
+    LaneStats* red = reinterpret_cast<LaneStats*>(lds + med_words);
     uint64_t& s_tile = *reinterpret_cast<uint64_t*>(red + BS / 64);
     const uint32_t tid = threadIdx.x;
     for (;;) {
@@ -652,7 +690,7 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
         const uint64_t rem = wd.evens - (uint64_t)tile_evens * lt;
         const uint32_t cnt = (uint32_t)(rem < tile_evens ? rem : tile_evens);
         const WinGeom g = window_geom(n0, (uint64_t)cnt + HALO + WIN_PAD);
-        sieve_tile<BS>(win32, med_i0, g.qv0, g.lead, 0, g.nbits, T, tid, r0 ? r0 + wd.r0off : nullptr, wd.rbase);
+        sieve_tile<BS>(win32, med_i0, g.qv0, g.lead, 0, g.nbits, T, tid, pr + wd.r0off, wd.rbase);
         LaneStats st = lane_stats_zero();
         mark_words<BS>(reinterpret_cast<const uint64_t*>(win32), n0, cnt, mp, T.sp,
                        verified ? verified + wd.word0 + lt * (tile_evens / 64) : nullptr, nullptr, sl, lo, st,
@@ -758,11 +796,21 @@ __global__ __launch_bounds__(256) void reduce_kernel(MarkParams mp, const StatRe
     }
 }
 
-// r0[wins[w].r0off + k] = wins[w].rbase mod sp[k], k < wins[w].np (grid.y = window)
-__global__ void residue_kernel(Tables T, const WinDesc* __restrict__ wins, uint32_t* __restrict__ r0) {
+// pr[wins[w].r0off + k] for k < wins[w].np (grid.y = window)
+// e = smallest b >= 0 with p | rbase + 2b: b = (-rbase mod p) / 2 when that is
+// even, else (-rbase mod p + p) / 2 (rbase odd, so rbase + 2b is the odd multiple).
+__global__ void residue_kernel(Tables T, const WinDesc* __restrict__ wins, PrimeRes* __restrict__ pr) {
     const WinDesc wd = wins[blockIdx.y];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < wd.np; k += gridDim.x * blockDim.x)
-        r0[wd.r0off + k] = mod_small(wd.rbase, T.sp[k], T.inv[k], T.c32[k]);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < wd.np; k += gridDim.x * blockDim.x) {
+        const uint32_t p = T.sp[k];
+        const uint32_t r = mod_small(wd.rbase, p, T.inv[k], T.c32[k]);
+        const uint32_t t = r ? p - r : 0;
+        PrimeRes e;
+        e.p = p;
+        e.e = (t & 1) ? (t + p) >> 1 : t >> 1;
+        e.inv = T.inv[k];
+        pr[wd.r0off + k] = e;
+    }
 }
 
 __global__ void fill_kernel(uint64_t* p, uint64_t words, uint64_t value) {
@@ -964,15 +1012,15 @@ static hipError_t allow_lds(K kernel, size_t bytes) {
 }
 
 hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uint32_t lead, uint64_t nbits,
-                               uint32_t tile_bits, uint64_t* out, int tail_ones, const uint32_t* r0) {
+                               uint32_t tile_bits, uint64_t* out, int tail_ones, const PrimeRes* pr) {
     if (nbits == 0) return hipSuccess;
     const uint64_t ntiles = (nbits + tile_bits - 1) / tile_bits;
-    if (ntiles > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    const size_t lds = (size_t)(tile_bits / 32 + 2) * 4 + (size_t)(T.n_med > NPAT ? T.n_med - NPAT : 1) * 4;
+    if (ntiles > 0x7FFFFFFFull || tile_bits > SIEVE_TILE_MAX || (tile_bits & 63)) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(T.n_med > NPAT ? T.n_med - NPAT : 1) * 4;
     hipError_t e = allow_lds(sieve_window_kernel<SIEVE_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sieve_window_kernel<SIEVE_BS>, dim3((uint32_t)ntiles), dim3(SIEVE_BS), lds, s, T, qv0, lead,
-                       nbits, tile_bits, out, tail_ones, r0);
+                       nbits, tile_bits, out, tail_ones, pr);
     return hipGetLastError();
 }
 
@@ -992,27 +1040,32 @@ hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams&
 }
 
 hipError_t launch_residues(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin, uint32_t max_np,
-                           uint32_t* r0) {
+                           PrimeRes* pr) {
     if (!nwin || !max_np) return hipSuccess;
     uint32_t bx = (max_np + 255) / 256;
     if (bx > 1024) bx = 1024;
-    hipLaunchKernelGGL(residue_kernel, dim3(bx, nwin), dim3(256), 0, s, T, wins, r0);
+    hipLaunchKernelGGL(residue_kernel, dim3(bx, nwin), dim3(256), 0, s, T, wins, pr);
     return hipGetLastError();
 }
 
+static size_t fused_dyn_lds(uint32_t n_med) {
+    return (size_t)(((n_med > NPAT ? n_med - NPAT : 1) + 1) & ~1u) * 4 + sizeof(LaneStats) * (FUSED_BS / 64) + 8;
+}
+
 size_t fused_lds_bytes(uint32_t tile_evens, uint32_t n_med) {
-    return (size_t)((tile_evens + HALO + WIN_PAD) / 32 + 2) * 4 +
-           (size_t)(((n_med > NPAT ? n_med - NPAT : 1) + 1) & ~1u) * 4 + sizeof(LaneStats) * (FUSED_BS / 64) + 8;
+    (void)tile_evens;   // the window is sized for the largest tile
+    return (size_t)FUSED_WIN_WORDS * 4 + fused_dyn_lds(n_med);
 }
 
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
-                        uint64_t* verified, SlowList sl, StatRec* rec, const uint32_t* r0) {
-    const size_t lds = fused_lds_bytes(tile_evens, T.n_med);
+                        uint64_t* verified, SlowList sl, StatRec* rec, const PrimeRes* pr) {
+    if (tile_evens > FUSED_TILE_MAX || (tile_evens & 63)) return hipErrorInvalidValue;
+    const size_t lds = fused_dyn_lds(T.n_med);
     hipError_t e = allow_lds(fused_kernel<FUSED_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fused_kernel<FUSED_BS>, dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin, ntiles,
-                       tile_evens, tile_counter, verified, sl, rec, r0);
+                       tile_evens, tile_counter, verified, sl, rec, pr);
     return hipGetLastError();
 }
 

@@ -14,6 +14,11 @@ constexpr int MARK_BS = 512;     // threads per plain-marking workgroup
 constexpr int FUSED_BS = 1024;   // threads per fused persistent workgroup
 constexpr int SERVE_WAVES = 4;   // queries per serving workgroup (one wave each)
 
+constexpr uint32_t SIEVE_TILE_MAX = 1u << 18;   // plain sieve: window bits per workgroup, at most
+constexpr uint32_t SIEVE_WIN_WORDS = SIEVE_TILE_MAX / 32 + 2;
+constexpr uint32_t FUSED_TILE_MAX = 1u << 20;   // fused: evens per tile, at most
+constexpr uint32_t FUSED_WIN_WORDS = (((FUSED_TILE_MAX + HALO + WIN_PAD) / 32 + 2) + 3) & ~3u;
+
 struct MarkParams {
     uint64_t p_small;
     uint64_t bound;       // max(p_small, phase2_limit)
@@ -39,10 +44,17 @@ struct WinDesc {
     uint64_t tile0;       // global index of the window's first tile
     uint64_t word0;       // first word of the window in the optional verified map
     uint64_t rbase;       // odd q every tile's q_lo is measured from (q_lo - rbase < 2^52)
-    uint64_t r0off;       // first entry of this window's residues (rbase mod sp[k])
-    uint32_t np;          // residues held: sieving primes any tile of the window uses
+    uint64_t r0off;       // first entry of this window's PrimeRes table
+    uint32_t np;          // PrimeRes entries held: sieving primes any tile of the window uses
     uint32_t pad;
     uint64_t pad2;
+};
+
+// A sieving prime as one window sees it: 16 bytes, one load per prime per tile.
+struct PrimeRes {
+    uint32_t p;
+    uint32_t e;     // first bit b >= 0 of the window's base q (rbase) with p | rbase + 2b
+    double inv;     // 1/p
 };
 
 // Sieving primes needed for windows whose largest q is q_top.
@@ -61,13 +73,13 @@ __host__ __device__ inline uint64_t sieve_prime_bound(uint64_t q_top) {
 // ---- launchers (stream-ordered, no synchronisation) ----
 
 // Odd-only sieve of window bits [0, nbits): bit i <-> q = qv0 + 2(i - lead), i >= lead.
-// r0 (nullable): qv0 mod sp[k] for every sieving prime the window needs.
+// pr: the window's sieving-prime entries (launch_residues with rbase = qv0).
 hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uint32_t lead, uint64_t nbits,
-                               uint32_t tile_bits, uint64_t* out, int tail_ones, const uint32_t* r0);
+                               uint32_t tile_bits, uint64_t* out, int tail_ones, const PrimeRes* pr);
 
-// r0[wins[w].r0off + k] = wins[w].rbase mod sp[k] for k < wins[w].np.
+// pr[wins[w].r0off + k] = sieving prime k as window w sees it, for k < wins[w].np.
 hipError_t launch_residues(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin, uint32_t max_np,
-                           uint32_t* r0);
+                           PrimeRes* pr);
 
 // Plain marking of one segment from a window already in device memory.
 hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams& mp, uint64_t n0, uint64_t cnt,
@@ -82,7 +94,7 @@ constexpr size_t LDS_PER_CU = 160 * 1024;
 // Fused persistent kernel over the tiles of `nwin` windows.
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
-                        uint64_t* verified, SlowList sl, StatRec* rec, const uint32_t* r0);
+                        uint64_t* verified, SlowList sl, StatRec* rec, const PrimeRes* pr);
 
 // Resolve slow-list entries: search primes above sp[K-1] up to bound.
 hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
