@@ -212,8 +212,55 @@ __device__ __forceinline__ bool sprp(uint64_t a, uint64_t d, uint32_t r, const M
     return false;
 }
 
-// Deterministic for every 64-bit n (bases 2..37; Sorenson & Webster 2015).
-// Small n (< 41) are answered directly; n even is composite.
+// Strong test to base 2: the multiply step of the ladder is a doubling.
+__device__ __forceinline__ bool sprp2(uint64_t d, uint32_t r, const Mont& M) {
+    uint64_t x = add_mod(M.one, M.one, M.n);   // 2 in Montgomery form
+    const int top = 63 - __clzll((long long)d);
+    for (int b = top - 1; b >= 0; --b) {
+        x = mont_mul(x, x, M);
+        if ((d >> b) & 1) x = add_mod(x, x, M.n);
+    }
+    const uint64_t minus_one = M.n - M.one;
+    if (x == M.one || x == minus_one) return true;
+    for (uint32_t i = 1; i < r; ++i) {
+        x = mont_mul(x, x, M);
+        if (x == minus_one) return true;
+        if (x == M.one) return false;
+    }
+    return false;
+}
+
+// The Miller-Rabin bases 2, 3, 5, ..., 37 in order.  The first 12 decide
+// every n < 2^64 (Sorenson & Webster 2015).  With fewer leading bases the
+// test is already exact below psi_k, the smallest odd composite passing the
+// first k of them (OEIS A014233), so mr_bases(n) of them give the 12-base answer.
+__device__ __forceinline__ uint32_t mr_base(uint32_t k) {
+    return k < 6 ? (0x0D0B07050302ull >> (8 * k)) & 0xFF : (0x251F1D171311ull >> (8 * (k - 6))) & 0xFF;
+}
+
+__host__ __device__ inline uint32_t mr_bases(uint64_t n) {
+    if (n < 2047ull) return 1;
+    if (n < 1373653ull) return 2;
+    if (n < 25326001ull) return 3;
+    if (n < 3215031751ull) return 4;
+    if (n < 2152302898747ull) return 5;
+    if (n < 3474749660383ull) return 6;
+    if (n < 341550071728321ull) return 7;
+    if (n < 3825123056546413051ull) return 9;
+    return 12;
+}
+
+// Strong test of odd n > 37 to the k-th base (k = 0: base 2).
+__device__ inline bool mr_test(uint64_t n, uint32_t k) {
+    const Mont M = mont_setup(n);
+    uint64_t d = n - 1;
+    const uint32_t r = __builtin_ctzll(d);
+    d >>= r;
+    return k == 0 ? sprp2(d, r, M) : sprp(mr_base(k), d, r, M);
+}
+
+// Deterministic for every 64-bit n.  Small n (< 41) are answered directly;
+// n even is composite.
 __device__ inline bool is_prime_u64(uint64_t n) {
     if (n < 41) return (0x20A08A28ACull >> n) & 1;   // primes 2..37 as a mask
     if ((n & 1) == 0) return false;
@@ -222,10 +269,11 @@ __device__ inline bool is_prime_u64(uint64_t n) {
     uint64_t d = n - 1;
     uint32_t r = __builtin_ctzll(d);
     d >>= r;
-    const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    if (!sprp2(d, r, M)) return false;
+    const uint32_t nb = mr_bases(n);
 #pragma unroll 1
-    for (int i = 0; i < 12; ++i)
-        if (!sprp(bases[i], d, r, M)) return false;
+    for (uint32_t i = 1; i < nb; ++i)
+        if (!sprp(mr_base(i), d, r, M)) return false;
     return true;
 }
 

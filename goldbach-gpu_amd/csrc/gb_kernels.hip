@@ -48,8 +48,6 @@ __device__ __forceinline__ uint32_t count_le(const uint32_t* sp, uint32_t n, uin
     return lo;
 }
 
-__device__ __noinline__ bool is_prime_u64_call(uint64_t q) { return is_prime_u64(q); }
-
 // Clearing bits of the LDS sieve window.  One LDS instruction per mark: a CU
 // retires about one wave64 LDS instruction per 4.5 clocks (ds_and_b32 and
 // plain stores alike; ds_mskor_b32 is slower).  Along a run i, i+p, i+2p, ...
@@ -86,6 +84,74 @@ __device__ __forceinline__ void clear_run_n(uint32_t* win32, uint32_t i, uint32_
 }
 
 // ---------------------------------------------------------------------------
+// Block-wide exclusive prefix sum of one count per thread (wave scans through
+// lane shuffles, wave totals through wsum[BS/64] in LDS).  *total = block sum.
+// ---------------------------------------------------------------------------
+template <int BS>
+__device__ uint32_t block_scan(uint32_t c, uint32_t tid, uint32_t* __restrict__ wsum, uint32_t* total) {
+    const uint32_t lane = tid & 63, wv = tid >> 6;
+    uint32_t x = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) {
+        const uint32_t v = wsum[w];
+        before += (uint32_t)w < wv ? v : 0;
+        all += v;
+    }
+    __syncthreads();   // wsum may be rewritten by the next scan
+    *total = all;
+    return before + x - c;
+}
+
+// Miller-Rabin decision of the pre-sieve survivors of a tile (bit i <-> odd
+// q = qb + 2i, every q > 2^44).  Survivors are packed into an LDS queue so all
+// lanes run equal-length exponentiations instead of each lane walking its own
+// word's survivors:
+//   pass 0: every survivor gets the base-2 strong test,
+//   pass 1: every (surviving q, base k) pair for k = 1 .. mr_bases(q_top)-1 is
+//           one lane's test;
+// a failed test clears the bit.  Extra bases never change a correct answer,
+// so the tile's largest q sets the base count for all of its q.
+template <int BS>
+__device__ void mr_tile(uint32_t* __restrict__ win32, uint32_t nbits, uint64_t qb, uint32_t tid,
+                        uint32_t* __restrict__ queue, uint32_t qcap, uint32_t* __restrict__ wsum) {
+    const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
+    const uint32_t nw64 = (nbits + 63) >> 6;
+    const uint32_t nb = mr_bases(qb + 2ull * (nbits - 1));
+    for (uint32_t pass = 0; pass < 2; ++pass) {
+        const uint32_t per = pass == 0 ? 1u : nb - 1;   // tests per queued q
+        if (per == 0) break;
+        for (uint32_t wb = 0; wb < nw64; wb += BS) {
+            const uint32_t k = wb + tid;
+            uint64_t w = k < nw64 ? win64[k] : 0;
+            if (k == nw64 - 1 && (nbits & 63)) w &= (1ull << (nbits & 63)) - 1;
+            uint32_t total;
+            const uint32_t off = block_scan<BS>((uint32_t)__popcll(w), tid, wsum, &total);
+            for (uint32_t base = 0; base < total; base += qcap) {
+                uint32_t pos = off;
+                for (uint64_t x = w; x; x &= x - 1, ++pos)
+                    if (pos - base < qcap) queue[pos - base] = 64 * k + __builtin_ctzll(x);
+                __syncthreads();
+                const uint32_t n = total - base < qcap ? total - base : qcap;
+                for (uint32_t u = tid; u < n * per; u += BS) {
+                    const uint32_t j = u / per;
+                    const uint32_t i = queue[j];
+                    if (!mr_test(qb + 2ull * i, pass == 0 ? 0u : u - j * per + 1)) lds_clear_bit(win32, i);
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // sieve_tile: window bits [g0, g0 + nbits) of a window whose bit i (>= lead)
 // holds q = qv0 + 2(i - lead).  Result in win32[0 .. ceil(nbits/32)).
 // pr[k] describes sieving prime sp[k] relative to the odd rbase <= q_lo
@@ -105,9 +171,10 @@ __device__ __forceinline__ uint32_t first_off(double y, uint32_t p, double inv) 
 }
 
 template <int BS>
-__device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ med_i0, uint64_t qv0,
-                           uint32_t lead, uint64_t g0, uint32_t nbits, const Tables& T, uint32_t tid,
-                           const PrimeRes* __restrict__ pr, uint64_t rbase) {
+__device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ scratch, uint32_t scratch_cap,
+                           uint32_t* __restrict__ wsum, uint64_t qv0, uint32_t lead, uint64_t g0, uint32_t nbits,
+                           const Tables& T, uint32_t tid, const PrimeRes* __restrict__ pr, uint64_t rbase) {
+    uint32_t* med_i0 = scratch;   // medium-prime first marks; the Miller-Rabin queue afterwards
     uint64_t* win64 = reinterpret_cast<uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
     const uint64_t b_lo = g0 > lead ? g0 : (uint64_t)lead;
@@ -227,20 +294,8 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
 
     // 4) above 2^44 the table cannot reach sqrt(q): survivors of the pre-sieve
     //    are decided by deterministic Miller-Rabin.
-    if (mr) {
-        for (uint32_t k = tid; k < nw64; k += BS) {
-            uint64_t w = win64[k], keep = w;
-            const uint64_t gb = g0 + 64ull * k;
-            while (w) {
-                const uint32_t b = __builtin_ctzll(w);
-                w &= w - 1;
-                const uint64_t gi = gb + b;
-                if (gi >= g0 + nbits) break;
-                if (!is_prime_u64_call(qv0 + 2ull * (gi - lead))) keep &= ~(1ull << b);
-            }
-            win64[k] = keep;
-        }
-        __syncthreads();
+    if (mr && any_valid) {
+        mr_tile<BS>(win32, nbits, qv0 + 2 * g0 - 2ull * lead, tid, scratch, scratch_cap, wsum);
     }
 }
 
@@ -603,13 +658,14 @@ __global__ __launch_bounds__(BS) void sieve_window_kernel(Tables T, uint64_t qv0
                                                           uint32_t tile_bits, uint64_t* __restrict__ out,
                                                           int tail_ones, const PrimeRes* __restrict__ pr) {
     __shared__ __align__(16) uint32_t win32[SIEVE_WIN_WORDS];   // fixed LDS address: folds into ds offsets
-    extern __shared__ uint32_t lds[];                             // medium-prime first marks
-    uint32_t* med_i0 = lds;
+    extern __shared__ uint32_t lds[];   // [scan words | medium first marks, then the Miller-Rabin queue]
+    uint32_t* wsum = lds;
+    uint32_t* scratch = lds + BS / 64;
     const uint32_t tid = threadIdx.x;
     const uint64_t g0 = (uint64_t)blockIdx.x * tile_bits;
     if (g0 >= nbits) return;
     const uint32_t tb = (uint32_t)(nbits - g0 < tile_bits ? nbits - g0 : tile_bits);
-    sieve_tile<BS>(win32, med_i0, qv0, lead, g0, tb, T, tid, pr, qv0);
+    sieve_tile<BS>(win32, scratch, SIEVE_MRQ, wsum, qv0, lead, g0, tb, T, tid, pr, qv0);
     const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
     const uint32_t nw = (tb + 63) >> 6;
     const uint64_t wo = g0 >> 6;
@@ -666,11 +722,11 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
     // the window at a fixed LDS address (sized for the largest tile); dynamic
     // LDS after it: [medium first marks | stats | tile id]
     __shared__ __align__(16) uint32_t win32[FUSED_WIN_WORDS];
-    extern __shared__ uint32_t lds[];
-    const uint32_t med_words = ((T.n_med > NPAT ? T.n_med - NPAT : 1) + 1) & ~1u;
-    uint32_t* med_i0 = lds;
-    LaneStats* red = reinterpret_cast<LaneStats*>(lds + med_words);
+    extern __shared__ uint32_t lds[];   // [stats | tile id | scan words | scratch]
+    LaneStats* red = reinterpret_cast<LaneStats*>(lds);
     uint64_t& s_tile = *reinterpret_cast<uint64_t*>(red + BS / 64);
+    uint32_t* wsum = reinterpret_cast<uint32_t*>(&s_tile + 1);
+    uint32_t* scratch = wsum + BS / 64;   // medium first marks, then the Miller-Rabin queue
     const uint32_t tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s_tile = atomicAdd((unsigned long long*)tile_counter, 1ull);
@@ -690,7 +746,7 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
         const uint64_t rem = wd.evens - (uint64_t)tile_evens * lt;
         const uint32_t cnt = (uint32_t)(rem < tile_evens ? rem : tile_evens);
         const WinGeom g = window_geom(n0, (uint64_t)cnt + HALO + WIN_PAD);
-        sieve_tile<BS>(win32, med_i0, g.qv0, g.lead, 0, g.nbits, T, tid, pr + wd.r0off, wd.rbase);
+        sieve_tile<BS>(win32, scratch, FUSED_MRQ, wsum, g.qv0, g.lead, 0, g.nbits, T, tid, pr + wd.r0off, wd.rbase);
         LaneStats st = lane_stats_zero();
         mark_words<BS>(reinterpret_cast<const uint64_t*>(win32), n0, cnt, mp, T.sp,
                        verified ? verified + wd.word0 + lt * (tile_evens / 64) : nullptr, nullptr, sl, lo, st,
@@ -887,8 +943,8 @@ __global__ void prime_aux_kernel(const uint32_t* __restrict__ sp, uint32_t n, do
 
 // ---------------------------------------------------------------------------
 // serving: one wave per query; lanes try 64 consecutive primes, base-2 SPRP
-// filters, the lowest surviving lane is confirmed with the 12 bases spread
-// over 12 lanes.  The answer is the minimal p (scan order of cpu_goldbach.cpp:19-22).
+// filters, the lowest surviving lane is confirmed with the remaining bases
+// spread over lanes 1 .. mr_bases(q)-1.  The answer is the minimal p (scan order of cpu_goldbach.cpp:19-22).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool small_factor(uint64_t q) {
     return q % 3 == 0 || q % 5 == 0 || q % 7 == 0 || q % 11 == 0 || q % 13 == 0 || q % 17 == 0 ||
@@ -923,9 +979,7 @@ __device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint6
         if (q <= T.p2_limit) {
             pass = is_prime_q(q, T);
         } else if (!small_factor(q)) {
-            const Mont M = mont_setup(q);
-            const uint32_t r = __builtin_ctzll(q - 1);
-            pass = sprp(2, (q - 1) >> r, r, M);
+            pass = mr_test(q, 0);
         }
     }
     uint64_t ball = __ballot(pass);
@@ -933,12 +987,7 @@ __device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint6
         const uint32_t L = __ffsll((unsigned long long)ball) - 1;
         const uint64_t qq = __shfl(q, L);
         bool ok = true;
-        if (qq > T.p2_limit && lane < 12) {
-            const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
-            const Mont M = mont_setup(qq);
-            const uint32_t r = __builtin_ctzll(qq - 1);
-            ok = sprp(bases[lane], (qq - 1) >> r, r, M);
-        }
+        if (qq > T.p2_limit && lane >= 1 && lane < mr_bases(qq)) ok = mr_test(qq, lane);
         if (__all(ok)) {
             res = __shfl(p, L);
             return 1;
@@ -1016,7 +1065,8 @@ hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uin
     if (nbits == 0) return hipSuccess;
     const uint64_t ntiles = (nbits + tile_bits - 1) / tile_bits;
     if (ntiles > 0x7FFFFFFFull || tile_bits > SIEVE_TILE_MAX || (tile_bits & 63)) return hipErrorInvalidValue;
-    const size_t lds = (size_t)(T.n_med > NPAT ? T.n_med - NPAT : 1) * 4;
+    const uint32_t med = T.n_med > NPAT ? T.n_med - NPAT : 1;
+    const size_t lds = (size_t)(SIEVE_BS / 64 + (med > SIEVE_MRQ ? med : SIEVE_MRQ)) * 4;
     hipError_t e = allow_lds(sieve_window_kernel<SIEVE_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sieve_window_kernel<SIEVE_BS>, dim3((uint32_t)ntiles), dim3(SIEVE_BS), lds, s, T, qv0, lead,
@@ -1049,7 +1099,8 @@ hipError_t launch_residues(hipStream_t s, const Tables& T, const WinDesc* wins, 
 }
 
 static size_t fused_dyn_lds(uint32_t n_med) {
-    return (size_t)(((n_med > NPAT ? n_med - NPAT : 1) + 1) & ~1u) * 4 + sizeof(LaneStats) * (FUSED_BS / 64) + 8;
+    const uint32_t med = n_med > NPAT ? n_med - NPAT : 1;
+    return sizeof(LaneStats) * (FUSED_BS / 64) + 8 + (size_t)(FUSED_BS / 64 + (med > FUSED_MRQ ? med : FUSED_MRQ)) * 4;
 }
 
 size_t fused_lds_bytes(uint32_t tile_evens, uint32_t n_med) {

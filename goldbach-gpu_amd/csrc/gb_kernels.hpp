@@ -18,6 +18,8 @@ constexpr uint32_t SIEVE_TILE_MAX = 1u << 18;   // plain sieve: window bits per 
 constexpr uint32_t SIEVE_WIN_WORDS = SIEVE_TILE_MAX / 32 + 2;
 constexpr uint32_t FUSED_TILE_MAX = 1u << 20;   // fused: evens per tile, at most
 constexpr uint32_t FUSED_WIN_WORDS = (((FUSED_TILE_MAX + HALO + WIN_PAD) / 32 + 2) + 3) & ~3u;
+constexpr uint32_t SIEVE_MRQ = 4096;   // Miller-Rabin queue entries of a plain sieve workgroup
+constexpr uint32_t FUSED_MRQ = 7168;   // ... of a fused workgroup (with the window: 157 KB of LDS)
 
 struct MarkParams {
     uint64_t p_small;

@@ -64,14 +64,16 @@ uint64_t orc_isqrt(uint64_t n) {
 /* ------------------------------------------------------------------------- */
 static inline uint64_t mulmod(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)((u128)a * b % m); }
 
-int orc_is_prime(uint64_t n) {
+/* n passes the strong test to each of the first k bases 2, 3, 5, ... (odd n > 3) */
+int orc_strong_first(uint64_t n, int k) {
     if (n < 2) return 0;
     if (n == 2 || n == 3) return 1;
     if ((n & 1) == 0) return 0;
     uint64_t d = n - 1, r = 0;
     while ((d & 1) == 0) { d >>= 1; r++; }
     static const uint64_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
-    for (int i = 0; i < 12; i++) {
+    if (k > 12) k = 12;
+    for (int i = 0; i < k; i++) {
         uint64_t a = bases[i];
         if (a >= n) continue;
         uint64_t x = 1, b = a % n, e = d;
@@ -89,6 +91,18 @@ int orc_is_prime(uint64_t n) {
         if (!ok) return 0;
     }
     return 1;
+}
+
+int orc_is_prime(uint64_t n) { return orc_strong_first(n, 12); }
+
+/* smallest odd composite in [9, limit] passing the first k bases, 0 if none */
+uint64_t orc_first_strong_pseudoprime(int k, uint64_t limit) {
+    for (uint64_t n = 9; n <= limit; n += 2) {
+        if (!orc_strong_first(n, k)) continue;
+        for (uint64_t f = 3; f * f <= n; f += 2)
+            if (n % f == 0) return n;
+    }
+    return 0;
 }
 
 /* exact primality, table first */

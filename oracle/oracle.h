@@ -70,6 +70,10 @@ uint64_t orc_window_word_count(uint64_t q_low, uint64_t q_high);
 int orc_window_bits(uint64_t q_low, uint64_t q_high, uint64_t *words);
 
 int orc_is_prime(uint64_t n);
+/* n passes the strong (Miller-Rabin) test to each of the first k of the bases 2..37 */
+int orc_strong_first(uint64_t n, int k);
+/* smallest odd composite in [9, limit] passing the first k bases (0 if none): brute force */
+uint64_t orc_first_strong_pseudoprime(int k, uint64_t limit);
 
 /* Smallest prime p <= min(bound, n/2) with n-p prime; 0 if none (or n odd / n < 4). */
 uint64_t orc_min_partition(uint64_t n, uint64_t bound);

@@ -55,6 +55,9 @@ class Oracle:
         L.orc_window_word_count.restype = u64; L.orc_window_word_count.argtypes = [u64, u64]
         L.orc_window_bits.restype = ctypes.c_int; L.orc_window_bits.argtypes = [u64, u64, vp]
         L.orc_is_prime.restype = ctypes.c_int; L.orc_is_prime.argtypes = [u64]
+        L.orc_strong_first.restype = ctypes.c_int; L.orc_strong_first.argtypes = [u64, ctypes.c_int]
+        L.orc_first_strong_pseudoprime.restype = u64
+        L.orc_first_strong_pseudoprime.argtypes = [ctypes.c_int, u64]
         L.orc_min_partition.restype = u64; L.orc_min_partition.argtypes = [u64, u64]
         L.orc_min_partition_batch.restype = None
         L.orc_min_partition_batch.argtypes = [vp, ctypes.c_size_t, u64, vp, ctypes.c_int]
@@ -88,6 +91,12 @@ class Oracle:
         out = np.zeros(max(nw, 1), dtype=np.uint64)
         assert self.L.orc_window_bits(q_low, q_high, out.ctypes.data) == 0
         return out[:nw]
+
+    def strong_first(self, n, k):
+        return bool(self.L.orc_strong_first(n, k))
+
+    def first_strong_pseudoprime(self, k, limit):
+        return int(self.L.orc_first_strong_pseudoprime(k, limit))
 
     def is_prime(self, n):
         return bool(self.L.orc_is_prime(n))

@@ -17,7 +17,18 @@ WINDOWS = [
     (17_592_177_655_809, 17_592_186_244_417),       # straddles 2^44: some tiles sieve, some test
     (10**18 + 1, 10**18 + 100_001),
     (2**64 - 2**21 + 1, 2**64 - 1),                 # top of the 64-bit range
+    (341550071728321 - 100_000, 341550071728321 + 100_000),          # strong pseudoprime to bases 2..19
+    (3825123056546413051 - 40_000, 3825123056546413051 + 40_000),    # ... to bases 2..31
 ]
+
+# strong pseudoprimes whose factors all exceed the 2^18 pre-sieve reach Miller-Rabin
+SPSP = [341550071728321]
+
+
+@pytest.mark.parametrize("q", SPSP)
+def test_strong_pseudoprime_is_cleared(ctx, q):
+    bits = ctx.sieve_window(q - 2 * 64, q + 2 * 64)
+    assert unpack(bits, 129)[64] == 0
 
 
 @pytest.mark.parametrize("q_low,q_high", WINDOWS)

@@ -189,6 +189,10 @@ def test_min_partition_matches_oracle(ctx, orc):
     got = ctx.min_partition(n)
     want = orc.min_partition_batch(n)
     assert np.array_equal(got, want)
+    # q = n - 3 a strong pseudoprime to the first 8 and to the first 11 bases
+    for psi in (341550071728321, 3825123056546413051, 2152302898747, 3474749660383):
+        m = np.array([psi + 3, psi + 5, psi + 7], dtype=np.uint64)
+        assert np.array_equal(ctx.min_partition(m), orc.min_partition_batch(m))
     # odd or too small: no partition reported
     assert list(ctx.min_partition([2, 3, 7, 0])) == [0, 0, 0, 0]
     # each result is the same alone and in the mix

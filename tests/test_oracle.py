@@ -141,6 +141,32 @@ def test_miller_rabin(orc, ref):
     assert not orc.is_prime(2**64 - 1) and not orc.is_prime(2**62 + 1)
 
 
+# psi_k (OEIS A014233): the smallest odd composite that is a strong probable prime to
+# each of the first k prime bases; below psi_k those k bases decide primality.  The
+# device stops after mr_bases(n) bases on this table.
+PSI = [(1, 2047), (2, 1373653), (3, 25326001), (4, 3215031751), (5, 2152302898747), (6, 3474749660383),
+       (7, 341550071728321), (8, 341550071728321), (9, 3825123056546413051), (10, 3825123056546413051),
+       (11, 3825123056546413051)]
+PSI_FACTORS = {2047: (23, 89), 1373653: (829, 1657), 25326001: (2251, 11251), 3215031751: (151, 751, 28351),
+               2152302898747: (6763, 10627, 29947), 3474749660383: (1303, 16927, 157543),
+               341550071728321: (10670053, 32010157), 3825123056546413051: (149491, 747451, 34233211)}
+
+
+def test_mr_base_count_thresholds(orc):
+    for k, psi in PSI:
+        f = PSI_FACTORS[psi]
+        assert int(np.prod([np.uint64(x) for x in f], dtype=np.uint64)) == psi and all(trial_division(x) for x in f)
+        assert orc.strong_first(psi, k), (k, psi)          # fools the first k bases ...
+        assert not orc.is_prime(psi)                          # ... and not all twelve
+    # where the k-base threshold moves, psi_k passes k bases but fails base k+1
+    for k, psi in ((1, 2047), (2, 1373653), (3, 25326001), (4, 3215031751), (5, 2152302898747),
+                   (6, 3474749660383), (8, 341550071728321), (11, 3825123056546413051)):
+        assert not orc.strong_first(psi, k + 1), (k, psi)
+    # minimality, by brute force, for the thresholds small enough to search
+    assert orc.first_strong_pseudoprime(1, 2047) == 2047
+    assert orc.first_strong_pseudoprime(2, 1373653) == 1373653
+
+
 def test_published_partitions_are_prime(orc):
     for n, p in PUBLISHED:
         assert orc.is_prime(p) and orc.is_prime(n - p)
