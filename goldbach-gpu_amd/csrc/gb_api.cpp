@@ -58,7 +58,8 @@ int fail(gb_ctx* c, int code, const std::string& msg) {
 }
 
 // Medium-prime schedule for tiles of at most `tb` bits: prime k with
-// p*CHUNK < tb is cut into ceil(tb/(CHUNK*p)) items of CHUNK marks.
+// p*CHUNK < tb is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK marks (an
+// item skips the multiples of 3, so its CHUNK marks span CHUNK_SPAN*p bits).
 int build_items(gb_ctx* c, uint32_t tb, ItemTable* out) {
     for (auto& it : c->items)
         if (it.tb == tb) { *out = it; return GB_OK; }
@@ -69,7 +70,7 @@ int build_items(gb_ctx* c, uint32_t tb, ItemTable* out) {
     start.assign(NPAT + 1, 0);
     while (k < c->nsp && (uint64_t)c->h_sp[k] * CHUNK < tb) {
         const uint32_t p = c->h_sp[k];
-        const uint32_t chunks = (uint32_t)((tb + (uint64_t)CHUNK * p - 1) / ((uint64_t)CHUNK * p));
+        const uint32_t chunks = (uint32_t)((tb + (uint64_t)CHUNK_SPAN * p - 1) / ((uint64_t)CHUNK_SPAN * p));
         if (chunks > 4096) return fail(c, GB_EINVAL, "tile too large for the medium-prime schedule");
         for (uint32_t ch = 0; ch < chunks; ++ch) {
             items.push_back((k << 12) | ch);

@@ -24,7 +24,9 @@ constexpr uint32_t HALO_P = 2 * HALO + 1;    // largest p served by the window: 
 constexpr uint32_t WIN_PAD = 64;             // extra window bits read by the last word
 constexpr uint32_t SP_MAX = 1u << 22;        // sieving-prime table bound (odd primes <= 2^22)
 constexpr int NPAT = 17;                     // odd primes 3..61 applied as word patterns
-constexpr uint32_t CHUNK = 32;               // marks per medium-prime work item
+constexpr uint32_t CHUNK = 32;               // marks per medium-prime work item ...
+constexpr uint32_t CHUNK_SPAN = 48;          // ... spanning 48p bits: odd multiples p*m with 3 | m are
+                                             // skipped (the pattern of 3 already cleared them)
 constexpr uint32_t MR_PRESIEVE = 1u << 18;   // pre-sieve bound for tiles decided by Miller-Rabin
 
 struct Tables {

@@ -222,15 +222,28 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
 
     // 2) first marks of the medium primes (split into CHUNK-mark items)
     const uint32_t kmed_end = np < T.n_med ? np : T.n_med;
+    //    An entry holds the first mark p*m0 with 3 not dividing m0 and, in bit
+    //    31, which of the strides p, 2p follows it: the odd m prime to 3 step
+    //    +2, +4 from m = 5 mod 6 and +4, +2 from m = 1 mod 6.
+    const uint32_t ql3 = (uint32_t)(q_lo % 3);
     for (uint32_t k = NPAT + tid; k < kmed_end; k += BS) {
         uint64_t i0 = nbits;
+        uint32_t two_p_first = 0;
         if (any_valid) {
             const PrimeRes e = pr[k];
             i0 = lb + first_off((double)e.e - dbf, e.p, e.inv);
             const uint64_t pp = (uint64_t)e.p * e.p;   // the first mark is p^2 or above
             if (q_lo < pp) i0 = max(i0, lb + ((pp - q_lo) >> 1));
+            if (i0 < nbits) {
+                const uint32_t q3 = (ql3 + 2 * ((uint32_t)(i0 - lb) % 3)) % 3;   // p*m0 mod 3
+                if (q3 == 0) {
+                    i0 += e.p;                                   // m0 + 2 = 5 mod 6
+                } else {
+                    two_p_first = (q3 * (e.p % 3)) % 3 == 1;     // m0 = 1 mod 6
+                }
+            }
         }
-        med_i0[k - NPAT] = i0 < nbits ? (uint32_t)i0 : nbits;
+        med_i0[k - NPAT] = (i0 < nbits ? (uint32_t)i0 : nbits) | (two_p_first << 31);
     }
     __syncthreads();
 
@@ -243,7 +256,7 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
         }
     }
 
-    // 3a) medium items: CHUNK marks each (unrolled when the chunk is whole);
+    // 3a) medium items: CHUNK marks each (unrolled when the item is whole);
     //     the next item's words are loaded before this item's marks issue
     const uint32_t n_items = kmed_end > NPAT ? T.item_start[kmed_end] : 0;
     {
@@ -258,11 +271,16 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
                 it_n = T.items[un];
                 p_n = T.items_p[un];
             }
-            const uint32_t i = med_i0[(it >> 12) - NPAT] + (it & 4095u) * (CHUNK * p);
-            if (i + (CHUNK - 1) * p < nbits)
-                clear_run_n<CHUNK>(win32, i, p);
-            else
-                clear_run(win32, i, p, nbits);
+            const uint32_t f = med_i0[(it >> 12) - NPAT];
+            const uint32_t i = (f & 0x7FFFFFFFu) + (it & 4095u) * (CHUNK_SPAN * p);
+            const uint32_t s1 = (f >> 31) ? 2 * p : p;   // the two runs: i + 3pt and i + s1 + 3pt
+            if (i + (CHUNK_SPAN - 1) * p < nbits) {
+                clear_run_n<CHUNK / 2>(win32, i, 3 * p);
+                clear_run_n<CHUNK / 2>(win32, i + s1, 3 * p);
+            } else {
+                clear_run(win32, i, 3 * p, nbits);
+                clear_run(win32, i + s1, 3 * p, nbits);
+            }
             u = un;
         }
     }
