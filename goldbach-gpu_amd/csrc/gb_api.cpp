@@ -32,6 +32,7 @@ constexpr uint32_t DEFAULT_TILE_LOG2 = 20;               // fused tile: 2^20 eve
 constexpr uint32_t SLOW_CAP = 1u << 20;
 constexpr uint32_t SEED_LIMIT = 4095;                    // trial-division seed (covers sqrt(2^22))
 constexpr uint32_t SP_TABLE_CAP = 300000;                // odd primes below 2^22: 295,946
+constexpr uint64_t WHEEL_N_MAX = (1ull << 44) - (1ull << 24);   // wheel windows stay where the table sieves q
 
 std::mutex g_open_mu;
 std::string g_open_err;
@@ -60,22 +61,29 @@ int fail(gb_ctx* c, int code, const std::string& msg) {
 // Medium-prime schedule for tiles of at most `tb` bits: prime k with
 // p*CHUNK < tb is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK marks (an
 // item skips the multiples of 3, so its CHUNK marks span CHUNK_SPAN*p bits).
-int build_items(gb_ctx* c, uint32_t tb, ItemTable* out) {
+//
+// Wheel form (wheel = true): tb is the bit count of each stream and a prime
+// gets ceil(tb/(CHUNK*p)) items in each of the two streams, encoded
+// (k << 13) | (stream << 12) | chunk; the streams hold no multiples of 3.
+int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel) {
     for (auto& it : c->items)
-        if (it.tb == tb) { *out = it; return GB_OK; }
+        if (it.tb == tb && it.wheel == (wheel ? 1u : 0u)) { *out = it; return GB_OK; }
     ItemTable t;
     t.tb = tb;
+    t.wheel = wheel ? 1u : 0u;
     std::vector<uint32_t> items, items_p, start;
     uint32_t k = NPAT;
     start.assign(NPAT + 1, 0);
     while (k < c->nsp && (uint64_t)c->h_sp[k] * CHUNK < tb) {
         const uint32_t p = c->h_sp[k];
-        const uint32_t chunks = (uint32_t)((tb + (uint64_t)CHUNK_SPAN * p - 1) / ((uint64_t)CHUNK_SPAN * p));
+        const uint64_t span = (uint64_t)(wheel ? CHUNK : CHUNK_SPAN) * p;
+        const uint32_t chunks = (uint32_t)((tb + span - 1) / span);
         if (chunks > 4096) return fail(c, GB_EINVAL, "tile too large for the medium-prime schedule");
-        for (uint32_t ch = 0; ch < chunks; ++ch) {
-            items.push_back((k << 12) | ch);
-            items_p.push_back(p);
-        }
+        for (uint32_t st = 0; st < (wheel ? 2u : 1u); ++st)
+            for (uint32_t ch = 0; ch < chunks; ++ch) {
+                items.push_back(wheel ? ((k << 13) | (st << 12) | ch) : ((k << 12) | ch));
+                items_p.push_back(p);
+            }
         ++k;
         start.push_back((uint32_t)items.size());
     }
@@ -100,6 +108,7 @@ Tables make_tables(const gb_ctx* c, const ItemTable& it) {
     T.sp = c->d_sp;
     T.inv = c->d_inv;
     T.c32 = c->d_c32;
+    T.d5 = c->d_d5;
     T.items = it.d_items;
     T.items_p = it.d_items_p;
     T.item_start = it.d_start;
@@ -186,7 +195,7 @@ void set_residues(const gb_ctx* c, WinDesc* wd, uint64_t rbase, uint64_t q_top, 
     wd->r0off = r0off;
     wd->np = count_le(c->h_sp, bound);
     wd->pad = 0;
-    wd->pad2 = 0;
+    wd->kbase = 0;
 }
 
 int ensure_wins(gb_ctx* c, uint32_t n) {
@@ -235,12 +244,13 @@ int build_tables(gb_ctx* c) {
     HCK(c, hipMemcpyAsync(&nseed, d_cnt, 4, hipMemcpyDeviceToHost, s));
     HCK(c, hipStreamSynchronize(s));
     if (nseed == 0 || nseed > 1024) return fail(c, GB_EDEVICE, "seed prime count out of range");
-    HCK(c, launch_prime_aux(s, d_seed, nseed, d_seed_inv, d_seed_c32));
+    HCK(c, launch_prime_aux(s, d_seed, nseed, d_seed_inv, d_seed_c32, nullptr));
 
     Tables seedT{};
     seedT.sp = d_seed;
     seedT.inv = d_seed_inv;
     seedT.c32 = d_seed_c32;
+    seedT.d5 = nullptr;
     seedT.items = nullptr;
     seedT.items_p = nullptr;
     seedT.item_start = nullptr;
@@ -270,7 +280,8 @@ int build_tables(gb_ctx* c) {
     if (c->nsp == 0 || c->nsp > SP_TABLE_CAP) return fail(c, GB_EDEVICE, "prime table build produced a bad count");
     HCK(c, dalloc(&c->d_inv, c->nsp));
     HCK(c, dalloc(&c->d_c32, c->nsp));
-    HCK(c, launch_prime_aux(s, c->d_sp, c->nsp, c->d_inv, c->d_c32));
+    HCK(c, dalloc(&c->d_d5, c->nsp));
+    HCK(c, launch_prime_aux(s, c->d_sp, c->nsp, c->d_inv, c->d_c32, c->d_d5));
     c->h_sp.resize(c->nsp);
     HCK(c, hipMemcpyAsync(c->h_sp.data(), c->d_sp, c->nsp * 4ull, hipMemcpyDeviceToHost, s));
     HCK(c, hipStreamSynchronize(s));
@@ -478,7 +489,7 @@ int gb_close(gb_ctx* c) {
         dfree(it.d_items_p);
         dfree(it.d_start);
     }
-    void* bufs[] = {c->d_sp, c->d_inv, c->d_c32, c->d_p2bits, c->d_window, c->d_verified, c->d_pmin, c->d_rec,
+    void* bufs[] = {c->d_sp, c->d_inv, c->d_c32, c->d_d5, c->d_p2bits, c->d_window, c->d_verified, c->d_pmin, c->d_rec,
                     c->d_out, c->d_slow_n, c->d_slow_win, c->d_slow_ctl, c->d_slow_res, c->d_wins, c->d_rec_lo,
                     c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0};
     for (void* b : bufs)
@@ -553,17 +564,43 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
     if (tile_log2 == 0) tile_log2 = DEFAULT_TILE_LOG2;
     if (tile_log2 < 12 || tile_log2 > 20) return fail(c, GB_EINVAL, "tile_log2 must be in [12, 20]");
     const uint32_t te = 1u << tile_log2;
-    ItemTable it;
-    int rc = build_items(c, te + HALO + WIN_PAD, &it);
-    if (rc) return rc;
-    const Tables T = make_tables(c, it);
     const MarkParams mp = mark_params(c, p_small, flags);
     const uint32_t nwin = (uint32_t)wins_in.size();
+    // The mod-6 wheel serves every window below 2^44 (where the table sieves q
+    // exactly) unless a verified map is wanted; its tiles hold 3 * 2^(tile_log2-1)
+    // even n.  Above, tiles pre-sieve and run Miller-Rabin on the odd-only window.
+    bool wheel = !verified_dev;
+    for (const WinDesc& w : wins_in)
+        if (w.start + 2 * (w.evens - 1) > WHEEL_N_MAX) wheel = false;
+    const uint32_t cls_len = te / 2;
+    ItemTable it;
+    int rc = wheel ? build_items(c, cls_len + WH_HALO + 64, &it, true) : build_items(c, te + HALO + WIN_PAD, &it);
+    if (rc) return rc;
+    const Tables T = make_tables(c, it);
     std::vector<WinDesc> wins(wins_in);
     std::vector<uint64_t> lo(nwin), hi(nwin);
     uint64_t tiles = 0, words = 0, nres = 0;
     uint32_t max_np = 0;
-    for (uint32_t w = 0; w < nwin; ++w) {
+    for (uint32_t w = 0; w < nwin && wheel; ++w) {
+        const uint64_t start = wins[w].start, n_hi = start + 2 * (wins[w].evens - 1);
+        const uint64_t nb = start - start % 6;
+        const uint64_t nt = (n_hi - nb) / (6ull * cls_len) + 1;
+        wins[w].tile0 = tiles;
+        wins[w].word0 = 0;
+        lo[w] = tiles;
+        tiles += nt;
+        hi[w] = tiles;
+        wins[w].kbase = (int64_t)(nb / 6) - (int64_t)WH_HALO;
+        const int64_t k_last = wins[w].kbase + (int64_t)(cls_len * (nt - 1));
+        const uint64_t q_top = 6ull * (uint64_t)(k_last + cls_len + WH_HALO + 64 - 1) + 5;
+        wins[w].rbase = 0;
+        wins[w].r0off = nres;
+        wins[w].np = count_le(c->h_sp, sieve_prime_bound(q_top));
+        wins[w].pad = 0;
+        nres += wins[w].np;
+        max_np = std::max(max_np, wins[w].np);
+    }
+    for (uint32_t w = 0; w < nwin && !wheel; ++w) {
         wins[w].tile0 = tiles;
         wins[w].word0 = words;
         lo[w] = tiles;
@@ -599,6 +636,7 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
     key.push_back(p_small);
     key.push_back(flags);
     key.push_back(tile_log2);
+    key.push_back(wheel ? 1 : 0);
     key.push_back(grid);
     key.push_back((uint64_t)(uintptr_t)verified_dev);
     key.push_back((uint64_t)(uintptr_t)c->d_rec);
@@ -612,10 +650,17 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
         HCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t e = launch_fill(s, c->d_slow_ctl, 2, 0);
         if (e == hipSuccess) e = launch_fill(s, c->d_counter, 1, 0);
-        if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, nwin, max_np, c->d_r0);
-        if (e == hipSuccess)
-            e = launch_fused(s, T, mp, c->d_wins, nwin, tiles, te, (uint32_t)grid, c->d_counter, verified_dev, sl,
-                             c->d_rec, c->d_r0);
+        if (wheel) {
+            if (e == hipSuccess) e = launch_residues_wheel(s, T, c->d_wins, nwin, max_np, c->d_r0);
+            if (e == hipSuccess)
+                e = launch_wheel(s, T, mp, c->d_wins, nwin, tiles, cls_len, (uint32_t)grid, c->d_counter, sl,
+                                 c->d_rec, c->d_r0);
+        } else {
+            if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, nwin, max_np, c->d_r0);
+            if (e == hipSuccess)
+                e = launch_fused(s, T, mp, c->d_wins, nwin, tiles, te, (uint32_t)grid, c->d_counter, verified_dev,
+                                 sl, c->d_rec, c->d_r0);
+        }
         if (e == hipSuccess) e = launch_slow(s, T, mp, sl, c->d_slow_res, c->d_wins, verified_dev, nullptr, 0);
         if (e == hipSuccess)
             e = launch_reduce(s, mp, c->d_rec, c->d_rec_lo, c->d_rec_hi, nwin, sl, c->d_slow_res, c->d_out);

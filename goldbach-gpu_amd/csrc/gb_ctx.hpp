@@ -19,6 +19,7 @@ constexpr uint32_t PLAIN_TILE_BITS = 1u << 18;   // plain sieve tile (bits per w
 
 struct ItemTable {
     uint32_t tb = 0;        // tile bits the schedule was cut for
+    uint32_t wheel = 0;     // 1: items per (prime, stream) of the mod-6 wheel
     uint32_t n_med = 0;
     uint32_t* d_items = nullptr;
     uint32_t* d_items_p = nullptr;
@@ -38,6 +39,7 @@ struct gb_ctx {
     uint32_t* d_sp = nullptr;
     double* d_inv = nullptr;
     uint32_t* d_c32 = nullptr;
+    uint32_t* d_d5 = nullptr;          // (-4 / 6) mod p per prime (wheel form)
     uint32_t nsp = 0;
     std::vector<uint32_t> h_sp;
     uint64_t* d_p2bits = nullptr;
@@ -105,7 +107,7 @@ hipError_t dalloc(T** p, uint64_t count) {
 }
 
 // Medium-prime schedule of the sieve for tiles of at most `tb` bits (cached on c).
-int build_items(gb_ctx* c, uint32_t tb, ItemTable* out);
+int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel = false);
 // Device table view of c with the schedule `it`.
 gbk::Tables make_tables(const gb_ctx* c, const ItemTable& it);
 

@@ -33,6 +33,7 @@ struct Tables {
     const uint32_t* sp;     // odd primes <= SP_MAX ascending (sp[0] = 3, sp[17] = 67)
     const double* inv;      // 1.0 / p (rounded)
     const uint32_t* c32;    // 2^32 mod p
+    const uint32_t* d5;     // (-4 / 6) mod p: from a multiple 6k+1 of p to the next 6k'+5 one, in k
     const uint32_t* items;  // medium-prime work items: (prime index << 12) | chunk
     const uint32_t* items_p;     // the prime of each item (saves a dependent load)
     const uint32_t* item_start;  // item_start[k] = first item of medium prime k

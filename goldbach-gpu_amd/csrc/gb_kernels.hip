@@ -388,8 +388,6 @@ __device__ void block_store(LaneStats s, uint64_t evens, StatRec* out, uint32_t 
 // compile-time constants: LDS reads carry immediate offsets (repeated words
 // are loaded once) and each prime costs two alignbit + two OR per lane.
 // ---------------------------------------------------------------------------
-constexpr int NMARK = 1027;   // odd primes in [3, 8193]
-
 struct MarkTable {
     uint16_t p[NMARK];
 };
@@ -517,13 +515,13 @@ __device__ __forceinline__ uint64_t word_valid(uint32_t w, uint32_t nw, uint32_t
 __device__ __forceinline__ void finish_word(uint32_t w, uint64_t v1, uint64_t v, uint64_t valid, uint64_t n_first,
                                             const MarkParams& mp, uint64_t* __restrict__ verified,
                                             uint32_t* __restrict__ pmin_out, const SlowList& sl, uint32_t win_idx,
-                                            LaneStats& st) {
+                                            LaneStats& st, uint32_t step = 2) {
     st.ph2 += __popcll(~v1 & valid);
     uint64_t u = ~v & valid;
     if (u) {
         if (mp.fast_final) {
             st.fail += __popcll(u);
-            const uint64_t n = n_first + 2ull * (64ull * w + __builtin_ctzll(u));
+            const uint64_t n = n_first + (uint64_t)step * (64ull * w + __builtin_ctzll(u));
             st.first = n < st.first ? n : st.first;
             if (pmin_out)
                 while (u) {
@@ -537,7 +535,7 @@ __device__ __forceinline__ void finish_word(uint32_t w, uint64_t v1, uint64_t v,
                 u &= u - 1;
                 const uint32_t slot = atomicAdd(sl.count, 1u);
                 if (slot < sl.cap) {
-                    sl.n[slot] = n_first + 2ull * (64ull * w + b);
+                    sl.n[slot] = n_first + (uint64_t)step * (64ull * w + b);
                     sl.win[slot] = win_idx;
                 } else {
                     atomicOr(sl.overflow, 1u);
@@ -773,6 +771,395 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
     }
 }
 
+// ===========================================================================
+// Wheel (mod 6) form of the fused fast path.
+//
+// The window keeps only q prime to 6, as two streams in LDS: bit i of stream
+// S1 / S5 holds q = 6(K0 + i) + 1 / + 5.  The even n of a tile are split by
+// n mod 6 into three classes, class c holding n = n0 + c + 6j (j < cls_len,
+// n0 = 6(K0 + WH_HALO)).  For a prime p the q = n - p of class c lie in stream
+// r = (c - p) mod 6 at the fixed bit offset WH_HALO + (c - p - r)/6 from j;
+// when r = 3 they are all multiples of 3 and class c skips p.  Against the
+// odd-only window this drops the multiples of 3 from every sieve pass, packs
+// 1.5x more n into the same LDS, and lets each class OR only the primes that
+// can give it a q prime to 6 (class 0 all of them, classes 2 and 4 half).
+// ===========================================================================
+
+__host__ __device__ constexpr uint32_t inv6_mod(uint32_t p) { return p % 6 == 1 ? (5 * p + 1) / 6 : (p + 1) / 6; }
+
+// first bit b >= 0 with p | 6(k + b) + r, from km = k mod p (p prime to 6)
+__device__ __forceinline__ uint32_t wheel_first(uint32_t km, uint32_t r, uint32_t p, uint32_t inv6) {
+    const uint32_t x = (6 * km + r) % p;
+    return (uint32_t)(((uint64_t)(x ? p - x : 0) * inv6) % p);
+}
+
+// First mark of the run off, off + p, ... in stream r (q = 6(K0 + i) + r) at or
+// above p^2: q = p must stay, and the multiples below p^2 are cleared by their
+// smaller factors anyway.
+__device__ __forceinline__ uint32_t wheel_p2(uint32_t off, uint32_t p, uint32_t r, int64_t K0, uint32_t end) {
+    const uint64_t pp = (uint64_t)p * p;
+    const int64_t kmin = (int64_t)((pp - r + 5) / 6);   // smallest k with 6k + r >= p^2
+    if (K0 + (int64_t)off >= kmin) return off;
+    const uint64_t gap = (uint64_t)(kmin - K0 - (int64_t)off);
+    const uint64_t i = off + (gap + p - 1) / p * p;
+    return i >= end ? end : (uint32_t)i;
+}
+
+template <int BS>
+__device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict__ med0, int64_t K0, uint32_t nbits,
+                                 const Tables& T, uint32_t tid, const PrimeRes* __restrict__ pr, int64_t kbase) {
+    const uint32_t nw64 = (nbits + 63) >> 6;
+    // 1) primes 5..61 as per-word masks in each stream; bits standing for q <= 1 are zero
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const uint32_t r = s ? 5u : 1u;
+        uint64_t* sw = reinterpret_cast<uint64_t*>(wh + s * WH_STREAM_WORDS);
+        uint32_t j0[NPAT - 1];
+#pragma unroll
+        for (int i = 1; i < NPAT; ++i) {
+            const uint32_t p = pat_p(i);
+            const uint32_t km = (uint32_t)(((uint64_t)(K0 + (int64_t)WH_HALO) % p + p - (WH_HALO % p)) % p);
+            const uint32_t kt = (km + (64u * tid) % p) % p;      // k of this thread's first bit, mod p
+            j0[i - 1] = wheel_first(kt, r, p, inv6_mod(p));
+        }
+        for (uint32_t k = tid; k < nw64; k += BS) {
+            uint64_t acc = 0;
+#pragma unroll
+            for (int i = 1; i < NPAT; ++i) {
+                const uint32_t p = pat_p(i);
+                acc |= pat_mask(p) << j0[i - 1];
+                const uint32_t t = j0[i - 1] - (64u * BS) % p;   // wraps high when negative
+                j0[i - 1] = min(t, t + p);
+            }
+            uint64_t w = ~acc;
+            const int64_t kw = K0 + 64 * (int64_t)k;             // k of the word's bit 0
+            if (kw <= 0) {                                        // q = 6k + r <= 1 below k = 0 (+ q = 1)
+                const int64_t zb = -kw + (r == 1 ? 1 : 0);        // low bits to clear
+                w = zb >= 64 ? 0 : (w & (~0ull << zb));
+            }
+            sw[k] = w;
+        }
+    }
+    // 2) first marks of the medium primes in both streams
+    const uint64_t q_top = 6ull * (uint64_t)(K0 + (int64_t)nbits - 1) + 5;
+    const uint32_t np = count_le(T.sp, T.nsp, isqrt64(q_top));
+    const uint32_t kmed_end = np < T.n_med ? np : T.n_med;
+    const double dbf = (double)(K0 - kbase);
+    for (uint32_t k = NPAT + tid; k < kmed_end; k += BS) {
+        const PrimeRes e = pr[k];
+        const uint32_t o1 = first_off((double)e.e - dbf, e.p, e.inv);
+        uint32_t o5 = o1 + T.d5[k];
+        o5 = min(o5, o5 - e.p);
+        med0[2 * (k - NPAT)] = wheel_p2(o1, e.p, 1, K0, nbits);
+        med0[2 * (k - NPAT) + 1] = wheel_p2(o5, e.p, 5, K0, nbits);
+    }
+    __syncthreads();
+    // pattern primes are themselves prime
+    if (tid >= 1 && tid < NPAT) {
+        const uint32_t p = pat_p(tid);
+        const int64_t i = (int64_t)(p / 6) - K0;
+        if (i >= 0 && i < (int64_t)nbits) atomicOr(&wh[(p % 6 == 5 ? WH_STREAM_WORDS : 0) + (uint32_t)(i >> 5)], 1u << (i & 31));
+    }
+    // 3a) medium items: CHUNK marks of one prime in one stream
+    const uint32_t n_items = kmed_end > NPAT ? T.item_start[kmed_end] : 0;
+    {
+        uint32_t u = tid, it_n = 0, p_n = 0;
+        if (u < n_items) {
+            it_n = T.items[u];
+            p_n = T.items_p[u];
+        }
+        while (u < n_items) {
+            const uint32_t it = it_n, p = p_n, un = u + BS;
+            if (un < n_items) {
+                it_n = T.items[un];
+                p_n = T.items_p[un];
+            }
+            const uint32_t sidx = (it >> 12) & 1u;
+            uint32_t* S = wh + sidx * WH_STREAM_WORDS;
+            const uint32_t i = med0[2 * ((it >> 13) - NPAT) + sidx] + (it & 4095u) * (CHUNK * p);
+            if (i + (CHUNK - 1) * p < nbits)
+                clear_run_n<CHUNK>(S, i, p);
+            else
+                clear_run(S, i, p, nbits);
+            u = un;
+        }
+    }
+    // 3b) large primes, one per thread, both streams
+    {
+        const int64_t kroot = (int64_t)(isqrt64(6ull * (uint64_t)(K0 > 0 ? K0 : 0) + 1) );
+        uint32_t k = T.n_med + tid;
+        PrimeRes nx{0, 0, 0.0};
+        uint32_t d_n = 0;
+        if (k < np) {
+            nx = pr[k];
+            d_n = T.d5[k];
+        }
+        while (k < np) {
+            const PrimeRes e = nx;
+            const uint32_t d5 = d_n, kn = k + BS;
+            if (kn < np) {
+                nx = pr[kn];
+                d_n = T.d5[kn];
+            }
+            uint32_t o1 = first_off((double)e.e - dbf, e.p, e.inv);
+            uint32_t o5 = o1 + d5;
+            o5 = min(o5, o5 - e.p);
+            if ((int64_t)e.p > kroot) {   // p^2 may lie above the tile's first q
+                o1 = wheel_p2(o1, e.p, 1, K0, nbits);
+                o5 = wheel_p2(o5, e.p, 5, K0, nbits);
+            }
+            clear_run(wh, o1, e.p, nbits);
+            clear_run(wh + WH_STREAM_WORDS, o5, e.p, nbits);
+            k = kn;
+        }
+    }
+    __syncthreads();
+}
+
+// marking of one class C (n = n_first + 6j): stream and offset per prime are compile-time
+template <int C, int K>
+__device__ __forceinline__ void wmark_one(const uint64_t* __restrict__ s1, const uint64_t* __restrict__ s5,
+                                          uint64_t& v) {
+    if constexpr (K < NMARK) {
+        constexpr int p = kMark.p[K];
+        constexpr int r = ((C - p) % 6 + 6) % 6;
+        if constexpr (r != 3) {
+            constexpr int off = (int)WH_HALO + (C - p - r) / 6;
+            static_assert(off >= 0, "halo covers every marking prime");
+            const uint64_t* S = r == 1 ? s1 : s5;
+            v |= funnel_c<(uint32_t)off & 63>(S[off >> 6], S[(off >> 6) + 1]);
+        }
+    }
+}
+
+template <int C, int K, int... I>
+__device__ __forceinline__ void wmark_group(const uint64_t* __restrict__ s1, const uint64_t* __restrict__ s5,
+                                            uint64_t& v, std::integer_sequence<int, I...>) {
+    (wmark_one<C, K + I>(s1, s5, v), ...);
+}
+
+template <int C, int K, int KEND, bool EXIT>
+__device__ __forceinline__ void wmark_span(const uint64_t* __restrict__ s1, const uint64_t* __restrict__ s5,
+                                           uint64_t& v) {
+    if constexpr (K < KEND && K < NMARK) {
+        wmark_group<C, K>(s1, s5, v, std::make_integer_sequence<int, 8>{});
+        if constexpr (K + 8 < KEND && K + 8 < NMARK) {
+            asm volatile("" ::: "memory");
+            if (!EXIT || !__all(v == ~0ull)) wmark_span<C, K + 8, KEND, EXIT>(s1, s5, v);
+        }
+    }
+}
+
+// compaction stage bounds in prime indices: class 0 can use every prime, classes 2 and 4 half of them
+template <int C> constexpr int WK1 = C == 0 ? 40 : 80;
+template <int C> constexpr int WK2 = C == 0 ? 72 : 144;
+
+// bits j of word w inside [jlo, jhi)
+__device__ __forceinline__ uint64_t range_mask(uint32_t w, uint32_t jlo, uint32_t jhi) {
+    const int64_t a = (int64_t)jlo - 64 * (int64_t)w, b = (int64_t)jhi - 64 * (int64_t)w;
+    const uint64_t hi = b >= 64 ? ~0ull : (b <= 0 ? 0 : ((1ull << b) - 1));
+    const uint64_t lo = a <= 0 ? ~0ull : (a >= 64 ? 0 : (~0ull << a));
+    return hi & lo;
+}
+
+template <int BS, int C>
+__device__ void wheel_mark_class(const uint64_t* __restrict__ s1, const uint64_t* __restrict__ s5, uint64_t n_first,
+                                 uint32_t nw, uint32_t jlo, uint32_t jhi, const MarkParams& mp, const SlowList& sl,
+                                 uint32_t win_idx, LaneStats& st, uint32_t tid) {
+    const uint32_t lane = tid & 63, wave = tid >> 6;
+    Pending P1, P2;
+    P1.n = 0; P2.n = 0;
+    P1.w0 = P1.w1 = P2.w0 = P2.w1 = 0;
+    P1.v0 = P1.v1 = P2.v0 = P2.v1 = ~0ull;
+    auto done = [&](uint32_t w, uint64_t v) {
+        finish_word(w, v, v, range_mask(w, jlo, jhi), n_first, mp, nullptr, nullptr, sl, win_idx, st, 6);
+    };
+    auto run_b = [&](uint32_t live) {
+        const bool ok = lane < live;
+        const uint32_t w = ok ? P1.w0 : 0;
+        uint64_t v = ok ? P1.v0 : ~0ull;
+        wmark_span<C, WK1<C>, WK2<C>, true>(s1 + w, s5 + w, v);
+        const bool unf = v != ~0ull;
+        if (ok && !unf) done(w, v);
+        pend_push(P2, unf, w, v, lane);
+    };
+    auto run_c = [&](uint32_t live) {
+        const bool ok = lane < live;
+        const uint32_t w = ok ? P2.w0 : 0;
+        uint64_t v = ok ? P2.v0 : ~0ull;
+        wmark_span<C, WK2<C>, NMARK, true>(s1 + w, s5 + w, v);
+        if (ok) done(w, v);
+    };
+    for (uint32_t base = wave * 64; base < nw; base += BS) {
+        const uint32_t w = base + lane;
+        const bool act = w < nw;
+        const uint32_t wr = act ? w : 0;
+        const uint64_t valid = act ? range_mask(w, jlo, jhi) : 0;
+        uint64_t v = ~valid;
+        if (act && w == 0) {
+            if (C == 4 && n_first == 4) v |= 1;   // 4 = 2 + 2
+            if (C == 0 && n_first == 0) v |= 2;   // 6 = 3 + 3 (q = 3 is not in the wheel)
+        }
+        wmark_span<C, 0, WK1<C>, false>(s1 + wr, s5 + wr, v);
+        const bool unf = v != ~0ull;
+        if (act && !unf) done(w, v);
+        pend_push(P1, unf, wr, v, lane);
+        if (P1.n >= 64) {
+            run_b(64);
+            pend_pop(P1);
+        }
+        if (P2.n >= 64) {
+            run_c(64);
+            pend_pop(P2);
+        }
+    }
+    if (P1.n) {
+        run_b(P1.n);
+        P1.n = 0;
+    }
+    while (P2.n) {
+        const uint32_t live = P2.n < 64 ? P2.n : 64;
+        run_c(live);
+        if (P2.n >= 64) pend_pop(P2);
+        else P2.n = 0;
+    }
+}
+
+// Table-driven marking of class c: p_min statistics, P_SMALL below the window
+// primes and the Phase-2 test hook (the unrolled path serves the rest).
+// Prime 3 gives class 0 only q = 3, i.e. n = 6, which the wheel does not hold.
+template <int BS>
+__device__ void wheel_mark_generic(const uint64_t* __restrict__ s1, const uint64_t* __restrict__ s5, uint64_t n0,
+                                   uint32_t c, uint32_t nw, uint32_t jlo, uint32_t jhi, const MarkParams& mp,
+                                   const uint32_t* __restrict__ sp, const SlowList& sl, uint32_t win_idx,
+                                   LaneStats& st, uint32_t tid) {
+    const bool want = mp.flags & 1u;
+    const uint32_t lane = tid & 63, wave = tid >> 6;
+    const uint64_t n_first = n0 + c;
+    for (uint32_t base = wave * 64; base < nw; base += BS) {
+        const uint32_t w = base + lane;
+        const bool act = w < nw;
+        const uint32_t wr = act ? w : 0;
+        const uint64_t valid = act ? range_mask(w, jlo, jhi) : 0;
+        uint64_t v = ~valid;
+        const bool has4 = act && c == 4 && w == 0 && n_first == 4 && (valid & 1);
+        if (has4 && mp.bound >= 2) {
+            v |= 1;   // 4 = 2 + 2, the only partition with p = 2
+            if (want) {
+                st.sum += 2;
+                merge_max(st.mx, st.arg, 2, 4);
+            }
+        }
+        uint64_t v1 = ~valid | ((has4 && mp.p_small >= 2) ? 1ull : 0ull);
+        bool crossed = mp.p_small < 3;
+        const bool has6 = act && c == 0 && w == 0 && n_first == 0;
+        for (uint32_t k = 0; k < mp.K; ++k) {
+            const uint32_t p = sp[k];
+            if (!crossed && p > mp.p_small) {
+                v1 = v;
+                crossed = true;
+            }
+            const uint32_t r = (c + 6 * 1366 - p) % 6;   // (c - p) mod 6, p <= 8191
+            uint64_t R;
+            if (r == 3) {
+                R = (has6 && p == 3) ? 2ull : 0ull;     // 6 = 3 + 3
+            } else {
+                const uint32_t off = WH_HALO - (p + r - c) / 6;
+                const uint64_t* S = (r == 1 ? s1 : s5) + wr + (off >> 6);
+                R = funnel64(S[0], S[1], off & 63);
+            }
+            if (want) {
+                const uint64_t nb = R & ~v;
+                if (nb) {
+                    st.sum += (uint64_t)__popcll(nb) * p;
+                    merge_max(st.mx, st.arg, p, n_first + 6ull * (64ull * w + __builtin_ctzll(nb)));
+                }
+            }
+            v |= R;
+            if ((k & 7) == 7 && __all(v == ~0ull)) break;
+        }
+        if (!crossed) v1 = v;
+        if (act) finish_word(w, v1, v, valid, n_first, mp, nullptr, nullptr, sl, win_idx, st, 6);
+    }
+}
+
+// j range of class c inside [n_lo, n_hi] for n = n0 + c + 6j, j < len
+__device__ __forceinline__ void class_range(uint64_t n0, uint32_t c, uint32_t len, uint64_t n_lo, uint64_t n_hi,
+                                            uint32_t& jlo, uint32_t& jhi) {
+    const uint64_t a = n0 + c;
+    jlo = a >= n_lo ? 0u : (uint32_t)min((n_lo - a + 5) / 6, (uint64_t)len);
+    jhi = a > n_hi ? 0u : (uint32_t)min((n_hi - a) / 6 + 1, (uint64_t)len);
+    if (jhi < jlo) jhi = jlo;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void wheel_kernel(Tables T, MarkParams mp, const WinDesc* __restrict__ wins,
+                                                   uint32_t nwin, uint64_t ntiles, uint32_t cls_len,
+                                                   uint64_t* __restrict__ tile_counter, SlowList sl,
+                                                   StatRec* __restrict__ rec, const PrimeRes* __restrict__ pr) {
+    __shared__ __align__(16) uint32_t wh[2 * WH_STREAM_WORDS];   // [S1 | S5] at fixed LDS addresses
+    extern __shared__ uint32_t lds[];                               // [stats | tile id | medium first marks]
+    LaneStats* red = reinterpret_cast<LaneStats*>(lds);
+    uint64_t& s_tile = *reinterpret_cast<uint64_t*>(red + BS / 64);
+    uint32_t* med0 = reinterpret_cast<uint32_t*>(&s_tile + 1);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nbits = cls_len + WH_HALO + 64;
+    const uint32_t nw = cls_len / 64;
+    const uint64_t* s1 = reinterpret_cast<const uint64_t*>(wh);
+    const uint64_t* s5 = reinterpret_cast<const uint64_t*>(wh + WH_STREAM_WORDS);
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd((unsigned long long*)tile_counter, 1ull);
+        __syncthreads();
+        const uint64_t t = s_tile;
+        __syncthreads();
+        if (t >= ntiles) break;
+        uint32_t lo = 0, hi = nwin;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (wins[mid].tile0 <= t) lo = mid; else hi = mid;
+        }
+        const WinDesc wd = wins[lo];
+        const uint64_t n_lo = wd.start, n_hi = wd.start + 2 * (wd.evens - 1);
+        const uint64_t n0 = (n_lo - n_lo % 6) + 6ull * cls_len * (t - wd.tile0);
+        const int64_t K0 = (int64_t)(n0 / 6) - (int64_t)WH_HALO;
+        sieve_wheel_tile<BS>(wh, med0, K0, nbits, T, tid, pr + wd.r0off, wd.kbase);
+        LaneStats st = lane_stats_zero();
+        uint32_t jlo0, jhi0, jlo2, jhi2, jlo4, jhi4;
+        class_range(n0, 0, cls_len, n_lo, n_hi, jlo0, jhi0);
+        class_range(n0, 2, cls_len, n_lo, n_hi, jlo2, jhi2);
+        class_range(n0, 4, cls_len, n_lo, n_hi, jlo4, jhi4);
+        if (mp.K == NMARK && mp.p_small >= HALO_P && !(mp.flags & 1u)) {
+            if (jhi0 > jlo0) wheel_mark_class<BS, 0>(s1, s5, n0, nw, jlo0, jhi0, mp, sl, lo, st, tid);
+            if (jhi2 > jlo2) wheel_mark_class<BS, 2>(s1, s5, n0 + 2, nw, jlo2, jhi2, mp, sl, lo, st, tid);
+            if (jhi4 > jlo4) wheel_mark_class<BS, 4>(s1, s5, n0 + 4, nw, jlo4, jhi4, mp, sl, lo, st, tid);
+        } else {
+            if (jhi0 > jlo0) wheel_mark_generic<BS>(s1, s5, n0, 0, nw, jlo0, jhi0, mp, T.sp, sl, lo, st, tid);
+            if (jhi2 > jlo2) wheel_mark_generic<BS>(s1, s5, n0, 2, nw, jlo2, jhi2, mp, T.sp, sl, lo, st, tid);
+            if (jhi4 > jlo4) wheel_mark_generic<BS>(s1, s5, n0, 4, nw, jlo4, jhi4, mp, T.sp, sl, lo, st, tid);
+        }
+        block_store<BS>(st, (uint64_t)(jhi0 - jlo0) + (jhi2 - jlo2) + (jhi4 - jlo4), &rec[t], tid, red);
+    }
+}
+
+// pr[wd.r0off + k] for the wheel: e = first bit of S1 (q = 6(kbase + e) + 1) holding a multiple of p
+__global__ void residue_wheel_kernel(Tables T, const WinDesc* __restrict__ wins, PrimeRes* __restrict__ pr) {
+    const WinDesc wd = wins[blockIdx.y];
+    const uint64_t kb = (uint64_t)(wd.kbase + (int64_t)WH_HALO);   // >= 0
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < wd.np; k += gridDim.x * blockDim.x) {
+        const uint32_t p = T.sp[k];
+        PrimeRes e;
+        e.p = p;
+        e.inv = T.inv[k];
+        if (p % 3 == 0) {
+            e.e = 0;
+        } else {
+            const uint32_t km = (mod_small(kb, p, T.inv[k], T.c32[k]) + p - (WH_HALO % p)) % p;   // kbase mod p
+            e.e = wheel_first(km, 1, p, inv6_mod(p));
+        }
+        pr[wd.r0off + k] = e;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // slow path: entries whose p_min exceeds the window's reach (p > sp[K-1])
 // ---------------------------------------------------------------------------
@@ -950,13 +1337,16 @@ __global__ __launch_bounds__(1024) void compact_primes_kernel(const uint64_t* __
     if (tid == 1023) *count_out = cnt[1023];
 }
 
+// d5 = (-4 * 6^-1) mod p, closed form: 6^-1 = (5p+1)/6 for p = 1 mod 6 and (p+1)/6
+// for p = 5 mod 6, giving (2p-2)/3 and (p-2)/3 (0 for p = 3, which the wheel never sieves).
 __global__ void prime_aux_kernel(const uint32_t* __restrict__ sp, uint32_t n, double* __restrict__ inv,
-                                 uint32_t* __restrict__ c32) {
+                                 uint32_t* __restrict__ c32, uint32_t* __restrict__ d5) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t p = sp[k];
     inv[k] = 1.0 / (double)p;
     c32[k] = (uint32_t)((1ull << 32) % p);
+    if (d5) d5[k] = p % 3 == 0 ? 0u : (p % 6 == 1 ? (2 * p - 2) / 3 : (p - 2) / 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -1138,6 +1528,33 @@ hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, co
     return hipGetLastError();
 }
 
+size_t wheel_lds_bytes(uint32_t n_med) {
+    const uint32_t med = n_med > NPAT ? n_med - NPAT : 1;
+    return (size_t)2 * WH_STREAM_WORDS * 4 + sizeof(LaneStats) * (FUSED_BS / 64) + 8 + (size_t)2 * med * 4;
+}
+
+hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin,
+                                 uint32_t max_np, PrimeRes* pr) {
+    if (!nwin || !max_np) return hipSuccess;
+    uint32_t bx = (max_np + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    hipLaunchKernelGGL(residue_wheel_kernel, dim3(bx, nwin), dim3(256), 0, s, T, wins, pr);
+    return hipGetLastError();
+}
+
+hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
+                        uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint64_t* tile_counter, SlowList sl,
+                        StatRec* rec, const PrimeRes* pr) {
+    if (cls_len > WH_CLASS_MAX || cls_len < 64 || (cls_len & 63)) return hipErrorInvalidValue;
+    const uint32_t med = T.n_med > NPAT ? T.n_med - NPAT : 1;
+    const size_t lds = sizeof(LaneStats) * (FUSED_BS / 64) + 8 + (size_t)2 * med * 4;
+    hipError_t e = allow_lds(wheel_kernel<FUSED_BS>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(wheel_kernel<FUSED_BS>, dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin, ntiles, cls_len,
+                       tile_counter, sl, rec, pr);
+    return hipGetLastError();
+}
+
 hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
                        const WinDesc* wins, uint64_t* verified, uint32_t* pmin_out, uint64_t pmin_n0) {
     hipLaunchKernelGGL(slow_kernel, dim3(256), dim3(256), 0, s, T, mp, sl, res_p, wins, verified, pmin_out, pmin_n0);
@@ -1172,8 +1589,8 @@ hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t n
     return hipGetLastError();
 }
 
-hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32) {
-    hipLaunchKernelGGL(prime_aux_kernel, dim3((n + 255) / 256), dim3(256), 0, s, sp, n, inv, c32);
+hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32, uint32_t* d5) {
+    hipLaunchKernelGGL(prime_aux_kernel, dim3((n + 255) / 256), dim3(256), 0, s, sp, n, inv, c32, d5);
     return hipGetLastError();
 }
 

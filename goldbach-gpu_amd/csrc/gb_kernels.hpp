@@ -13,6 +13,7 @@ constexpr int SIEVE_BS = 512;    // threads per plain-sieve workgroup
 constexpr int MARK_BS = 512;     // threads per plain-marking workgroup
 constexpr int FUSED_BS = 1024;   // threads per fused persistent workgroup
 constexpr int SERVE_WAVES = 4;   // queries per serving workgroup (one wave each)
+constexpr int NMARK = 1027;      // odd primes in [3, 8193]: the primes the window answers
 
 constexpr uint32_t SIEVE_TILE_MAX = 1u << 18;   // plain sieve: window bits per workgroup, at most
 constexpr uint32_t SIEVE_WIN_WORDS = SIEVE_TILE_MAX / 32 + 2;
@@ -49,7 +50,7 @@ struct WinDesc {
     uint64_t r0off;       // first entry of this window's PrimeRes table
     uint32_t np;          // PrimeRes entries held: sieving primes any tile of the window uses
     uint32_t pad;
-    uint64_t pad2;
+    int64_t kbase;        // wheel form: k of the first tile's stream bit 0 (q = 6k + 1 or 6k + 5)
 };
 
 // A sieving prime as one window sees it: 16 bytes, one load per prime per tile.
@@ -98,6 +99,20 @@ hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, co
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
                         uint64_t* verified, SlowList sl, StatRec* rec, const PrimeRes* pr);
 
+// Wheel (mod 6) form of the fused fast path: tiles of 3 * cls_len even n (cls_len per
+// class n mod 6), the window kept as two streams of q = 6k + 1 and q = 6k + 5.
+constexpr uint32_t WH_HALO = 1408;                   // k below n0 / 6 (covers p <= 8191: 1366)
+constexpr uint32_t WH_CLASS_MAX = 1u << 19;          // n per class per tile, at most
+constexpr uint32_t WH_STREAM_WORDS = (WH_CLASS_MAX + WH_HALO + 64) / 32 + 2;   // u32 per stream
+size_t wheel_lds_bytes(uint32_t n_med);
+// pr[wins[w].r0off + k]: prime k as window w's stream S1 sees it (e = first stream bit of a
+// multiple of p, counted from k = kbase).
+hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin,
+                                 uint32_t max_np, PrimeRes* pr);
+hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
+                        uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint64_t* tile_counter, SlowList sl,
+                        StatRec* rec, const PrimeRes* pr);
+
 // Resolve slow-list entries: search primes above sp[K-1] up to bound.
 hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
                        const WinDesc* wins, uint64_t* verified, uint32_t* pmin_out, uint64_t pmin_n0);
@@ -114,7 +129,7 @@ hipError_t launch_fill(hipStream_t s, uint64_t* p, uint64_t words, uint64_t valu
 hipError_t launch_seed_primes(hipStream_t s, uint32_t limit, uint64_t* bits_out);
 hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t nwords, uint32_t qlimit,
                                  uint32_t* primes_out, uint32_t cap, uint32_t* count_out);
-hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32);
+hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32, uint32_t* d5);
 
 // Serving step records: query n resumes its scan at candidate index k.
 struct ServeIn {
