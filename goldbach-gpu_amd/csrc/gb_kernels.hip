@@ -954,6 +954,11 @@ __device__ __forceinline__ void wmark_span(const uint64_t* __restrict__ s1, cons
 template <int C> constexpr int WK1 = C == 0 ? 40 : 80;
 template <int C> constexpr int WK2 = C == 0 ? 72 : 144;
 
+// Word-0 bit of n = 6 in class 0 (n = n_first + 6j): the tile base n0 is 0 or 6 there.
+__device__ __forceinline__ uint64_t six_bit(uint64_t n_first) {
+    return n_first == 0 ? 2ull : (n_first == 6 ? 1ull : 0ull);
+}
+
 // bits j of word w inside [jlo, jhi)
 __device__ __forceinline__ uint64_t range_mask(uint32_t w, uint32_t jlo, uint32_t jhi) {
     const int64_t a = (int64_t)jlo - 64 * (int64_t)w, b = (int64_t)jhi - 64 * (int64_t)w;
@@ -998,7 +1003,7 @@ __device__ void wheel_mark_class(const uint64_t* __restrict__ s1, const uint64_t
         uint64_t v = ~valid;
         if (act && w == 0) {
             if (C == 4 && n_first == 4) v |= 1;   // 4 = 2 + 2
-            if (C == 0 && n_first == 0) v |= 2;   // 6 = 3 + 3 (q = 3 is not in the wheel)
+            if (C == 0) v |= six_bit(n_first);    // 6 = 3 + 3 (q = 3 is not in the wheel)
         }
         wmark_span<C, 0, WK1<C>, false>(s1 + wr, s5 + wr, v);
         const bool unf = v != ~0ull;
@@ -1052,7 +1057,7 @@ __device__ void wheel_mark_generic(const uint64_t* __restrict__ s1, const uint64
         }
         uint64_t v1 = ~valid | ((has4 && mp.p_small >= 2) ? 1ull : 0ull);
         bool crossed = mp.p_small < 3;
-        const bool has6 = act && c == 0 && w == 0 && n_first == 0;
+        const uint64_t bit6 = (act && c == 0 && w == 0) ? six_bit(n_first) : 0;
         for (uint32_t k = 0; k < mp.K; ++k) {
             const uint32_t p = sp[k];
             if (!crossed && p > mp.p_small) {
@@ -1062,7 +1067,7 @@ __device__ void wheel_mark_generic(const uint64_t* __restrict__ s1, const uint64
             const uint32_t r = (c + 6 * 1366 - p) % 6;   // (c - p) mod 6, p <= 8191
             uint64_t R;
             if (r == 3) {
-                R = (has6 && p == 3) ? 2ull : 0ull;     // 6 = 3 + 3
+                R = p == 3 ? bit6 : 0ull;               // 6 = 3 + 3
             } else {
                 const uint32_t off = WH_HALO - (p + r - c) / 6;
                 const uint64_t* S = (r == 1 ? s1 : s5) + wr + (off >> 6);
