@@ -62,9 +62,12 @@ int fail(gb_ctx* c, int code, const std::string& msg) {
 // p*CHUNK < tb is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK marks (an
 // item skips the multiples of 3, so its CHUNK marks span CHUNK_SPAN*p bits).
 //
-// Wheel form (wheel = true): tb is the bit count of each stream and a prime
-// gets ceil(tb/(CHUNK*p)) items in each of the two streams, encoded
-// (k << 13) | (stream << 12) | chunk; the streams hold no multiples of 3.
+// Wheel form (wheel = true): tb is the bit count of each stream; items are
+// encoded (k << 13) | (stream << 12) | chunk and the streams hold no multiples
+// of 3.  Primes with at least 4 * 64 * WAVE_MARKS marks per stream (p < tb/2048)
+// get wave items: ceil(tb/(64*WAVE_MARKS*p)) per stream, one wavefront each,
+// lane l marking l, l + 64, ... of the item's 64*WAVE_MARKS marks; the others
+// get ceil(tb/(CHUNK*p)) lane items of CHUNK consecutive marks per stream.
 int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel) {
     for (auto& it : c->items)
         if (it.tb == tb && it.wheel == (wheel ? 1u : 0u)) { *out = it; return GB_OK; }
@@ -74,9 +77,12 @@ int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel) {
     std::vector<uint32_t> items, items_p, start;
     uint32_t k = NPAT;
     start.assign(NPAT + 1, 0);
+    t.k_wave = NPAT;
     while (k < c->nsp && (uint64_t)c->h_sp[k] * CHUNK < tb) {
         const uint32_t p = c->h_sp[k];
-        const uint64_t span = (uint64_t)(wheel ? CHUNK : CHUNK_SPAN) * p;
+        const bool wave_item = wheel && (uint64_t)p * 2048 < tb;
+        if (wave_item) t.k_wave = k + 1;
+        const uint64_t span = (uint64_t)(wave_item ? 64 * WAVE_MARKS : (wheel ? CHUNK : CHUNK_SPAN)) * p;
         const uint32_t chunks = (uint32_t)((tb + span - 1) / span);
         if (chunks > 4096) return fail(c, GB_EINVAL, "tile too large for the medium-prime schedule");
         for (uint32_t st = 0; st < (wheel ? 2u : 1u); ++st)
@@ -117,6 +123,7 @@ Tables make_tables(const gb_ctx* c, const ItemTable& it) {
     T.cover = SP_MAX;
     T.nsp = c->nsp;
     T.n_med = it.n_med;
+    T.k_wave = it.k_wave;
     return T;
 }
 
@@ -259,6 +266,7 @@ int build_tables(gb_ctx* c) {
     seedT.cover = SEED_LIMIT;
     seedT.nsp = nseed;
     seedT.n_med = NPAT;
+    seedT.k_wave = NPAT;
 
     const uint64_t sp_bits = (SP_MAX - 1 - 3) / 2 + 1;   // odd q in [3, 2^22 - 1]
     const uint64_t sp_words = (sp_bits + 63) / 64;

@@ -21,6 +21,7 @@ struct ItemTable {
     uint32_t tb = 0;        // tile bits the schedule was cut for
     uint32_t wheel = 0;     // 1: items per (prime, stream) of the mod-6 wheel
     uint32_t n_med = 0;
+    uint32_t k_wave = 0;    // wheel: primes below index k_wave are cut into wave items
     uint32_t* d_items = nullptr;
     uint32_t* d_items_p = nullptr;
     uint32_t* d_start = nullptr;

@@ -25,6 +25,7 @@ constexpr uint32_t WIN_PAD = 64;             // extra window bits read by the la
 constexpr uint32_t SP_MAX = 1u << 22;        // sieving-prime table bound (odd primes <= 2^22)
 constexpr int NPAT = 17;                     // odd primes 3..61 applied as word patterns
 constexpr uint32_t CHUNK = 32;               // marks per medium-prime work item ...
+constexpr uint32_t WAVE_MARKS = 8;           // marks per lane of a wheel wave item (64 lanes: 512 marks)
 constexpr uint32_t CHUNK_SPAN = 48;          // ... spanning 48p bits: odd multiples p*m with 3 | m are
                                              // skipped (the pattern of 3 already cleared them)
 constexpr uint32_t MR_PRESIEVE = 1u << 18;   // pre-sieve bound for tiles decided by Miller-Rabin
@@ -42,6 +43,7 @@ struct Tables {
     uint64_t cover;         // sp holds every odd prime <= cover
     uint32_t nsp;           // entries in sp
     uint32_t n_med;         // medium primes: sp[17 .. n_med) are split into items
+    uint32_t k_wave;        // wheel: sp[17 .. k_wave) use wave items (lanes interleave one run)
 };
 
 // Window geometry for a window starting at even n0 and covering `want` bits.

@@ -860,10 +860,35 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
         const int64_t i = (int64_t)(p / 6) - K0;
         if (i >= 0 && i < (int64_t)nbits) atomicOr(&wh[(p % 6 == 5 ? WH_STREAM_WORDS : 0) + (uint32_t)(i >> 5)], 1u << (i & 31));
     }
-    // 3a) medium items: CHUNK marks of one prime in one stream
+    // 3a) wave items: a wavefront takes 64 * WAVE_MARKS marks of one prime in
+    //     one stream, lane l the marks l, l + 64, ...  Those sit 64p bits apart:
+    //     the same bit of words 2p apart, so a lane's clear-mask never changes
+    //     and a mark costs one address add; the lanes' words lie ~p/32 apart.
+    const uint32_t kw_end = kmed_end < T.k_wave ? kmed_end : T.k_wave;
+    const uint32_t n_wave = kw_end > NPAT ? T.item_start[kw_end] : 0;
+    {
+        const uint32_t lane = tid & 63;
+        for (uint32_t u = __builtin_amdgcn_readfirstlane(tid >> 6); u < n_wave; u += BS / 64) {
+            const uint32_t it = T.items[u], p = T.items_p[u];
+            const uint32_t sidx = (it >> 12) & 1u;
+            uint32_t* S = wh + sidx * WH_STREAM_WORDS;
+            const uint32_t i0 = med0[2 * ((it >> 13) - NPAT) + sidx];
+            const uint32_t first = (it & 4095u) * (64 * WAVE_MARKS);   // the item's first mark number
+            uint32_t b = i0 + (first + lane) * p;
+            const uint32_t mask = clear_mask(b);
+            if (i0 + (first + 64 * WAVE_MARKS - 1) * p < nbits) {
+                uint32_t* wp = S + (b >> 5);
+#pragma unroll
+                for (uint32_t t = 0; t < WAVE_MARKS; ++t) atomicAnd(wp + 2 * p * t, mask);
+            } else {
+                for (uint32_t t = 0; t < WAVE_MARKS && b < nbits; ++t, b += 64 * p) atomicAnd(&S[b >> 5], mask);
+            }
+        }
+    }
+    // 3b) lane items: CHUNK consecutive marks of one prime in one stream per lane
     const uint32_t n_items = kmed_end > NPAT ? T.item_start[kmed_end] : 0;
     {
-        uint32_t u = tid, it_n = 0, p_n = 0;
+        uint32_t u = n_wave + tid, it_n = 0, p_n = 0;
         if (u < n_items) {
             it_n = T.items[u];
             p_n = T.items_p[u];
@@ -884,7 +909,7 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
             u = un;
         }
     }
-    // 3b) large primes, one per thread, both streams
+    // 3c) large primes, one per thread, both streams
     {
         const int64_t kroot = (int64_t)(isqrt64(6ull * (uint64_t)(K0 > 0 ? K0 : 0) + 1) );
         uint32_t k = T.n_med + tid;
@@ -950,9 +975,11 @@ __device__ __forceinline__ void wmark_span(const uint64_t* __restrict__ s1, cons
     }
 }
 
-// compaction stage bounds in prime indices: class 0 can use every prime, classes 2 and 4 half of them
-template <int C> constexpr int WK1 = C == 0 ? 40 : 80;
-template <int C> constexpr int WK2 = C == 0 ? 72 : 144;
+// Compaction stage bounds in prime indices (class 0 can use every prime, class
+// 2 the primes 1 mod 6, class 4 those 5 mod 6), set by a cost model of the
+// stages on the minimal partitions of 3 x 96,000 n near 1e12.
+template <int C> constexpr int WK1 = C == 0 ? 40 : (C == 2 ? 72 : 56);
+template <int C> constexpr int WK2 = C == 0 ? 56 : (C == 2 ? 112 : 88);
 
 // Word-0 bit of n = 6 in class 0 (n = n_first + 6j): the tile base n0 is 0 or 6 there.
 __device__ __forceinline__ uint64_t six_bit(uint64_t n_first) {
