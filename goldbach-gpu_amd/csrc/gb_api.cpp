@@ -80,7 +80,7 @@ int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel) {
     t.k_wave = NPAT;
     while (k < c->nsp && (uint64_t)c->h_sp[k] * CHUNK < tb) {
         const uint32_t p = c->h_sp[k];
-        const bool wave_item = wheel && (uint64_t)p * 2048 < tb;
+        const bool wave_item = wheel && (uint64_t)p * 4 * 64 * WAVE_MARKS < tb;
         if (wave_item) t.k_wave = k + 1;
         const uint64_t span = (uint64_t)(wave_item ? 64 * WAVE_MARKS : (wheel ? CHUNK : CHUNK_SPAN)) * p;
         const uint32_t chunks = (uint32_t)((tb + span - 1) / span);
