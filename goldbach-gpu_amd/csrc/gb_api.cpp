@@ -29,6 +29,7 @@ constexpr uint64_t DEFAULT_P2 = 100000000ull;           // goldbach.cu:316
 constexpr uint64_t VRAM_MARGIN = 50ull << 20;            // goldbach.cu:96
 constexpr uint32_t MARK_WPB = 2048;                      // plain marking: verified words per workgroup
 constexpr uint32_t DEFAULT_TILE_LOG2 = 20;               // fused tile: 2^20 evens
+constexpr uint32_t WHEEL_TILE_LOG2 = 19;                 // wheel default: 3 * 2^18 evens, two workgroups per CU
 constexpr uint32_t SLOW_CAP = 1u << 20;
 constexpr uint32_t SEED_LIMIT = 4095;                    // trial-division seed (covers sqrt(2^22))
 constexpr uint32_t SP_TABLE_CAP = 300000;                // odd primes below 2^22: 295,946
@@ -569,6 +570,13 @@ int gb_verify_segment(gb_ctx* c, uint64_t A, uint64_t cnt, uint64_t p_small, uin
 static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t p_small, uint32_t flags,
                        uint32_t tile_log2, uint32_t wg_per_cu, uint64_t* verified_dev, double* ms) {
     hipStream_t s = c->stream;
+    bool below = !verified_dev;   // every window where the wheel applies
+    for (const WinDesc& w : wins_in)
+        if (w.start + 2 * (w.evens - 1) > WHEEL_N_MAX) below = false;
+    if (tile_log2 == 0 && wg_per_cu == 0 && below) {
+        tile_log2 = WHEEL_TILE_LOG2;   // measured at [4, 1e12]: 181 ms against 198 ms for one 2^20 tile per CU
+        wg_per_cu = 2;
+    }
     if (tile_log2 == 0) tile_log2 = DEFAULT_TILE_LOG2;
     if (tile_log2 < 12 || tile_log2 > 20) return fail(c, GB_EINVAL, "tile_log2 must be in [12, 20]");
     const uint32_t te = 1u << tile_log2;
@@ -577,10 +585,10 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
     // The mod-6 wheel serves every window below 2^44 (where the table sieves q
     // exactly) unless a verified map is wanted; its tiles hold 3 * 2^(tile_log2-1)
     // even n.  Above, tiles pre-sieve and run Miller-Rabin on the odd-only window.
-    bool wheel = !verified_dev;
-    for (const WinDesc& w : wins_in)
-        if (w.start + 2 * (w.evens - 1) > WHEEL_N_MAX) wheel = false;
+    const bool wheel = below;
     const uint32_t cls_len = te / 2;
+    if (wheel && wg_per_cu >= 2 && cls_len > WH_CLASS_HALF)
+        return fail(c, GB_EINVAL, "two workgroups per CU take tiles of at most 2^19 evens");
     ItemTable it;
     int rc = wheel ? build_items(c, cls_len + WH_HALO + 64, &it, true) : build_items(c, te + HALO + WIN_PAD, &it);
     if (rc) return rc;
@@ -661,8 +669,8 @@ static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t 
         if (wheel) {
             if (e == hipSuccess) e = launch_residues_wheel(s, T, c->d_wins, nwin, max_np, c->d_r0);
             if (e == hipSuccess)
-                e = launch_wheel(s, T, mp, c->d_wins, nwin, tiles, cls_len, (uint32_t)grid, c->d_counter, sl,
-                                 c->d_rec, c->d_r0);
+                e = launch_wheel(s, T, mp, c->d_wins, nwin, tiles, cls_len, (uint32_t)grid, wg_per_cu, c->d_counter,
+                                 sl, c->d_rec, c->d_r0);
         } else {
             if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, nwin, max_np, c->d_r0);
             if (e == hipSuccess)
@@ -745,8 +753,8 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
     gb_tuning def{};
     def.form = form;
     def.flags = flags;
-    def.tile_log2 = DEFAULT_TILE_LOG2;
-    def.wg_per_cu = 1;
+    def.tile_log2 = limit <= WHEEL_N_MAX ? WHEEL_TILE_LOG2 : DEFAULT_TILE_LOG2;
+    def.wg_per_cu = limit <= WHEEL_N_MAX ? 2 : 1;
     def.seg_evens = std::min<uint64_t>(DEFAULT_SEG, c->cfg.max_seg_evens);
     const uint64_t evens = (limit - start) / 2 + 1;
     // four sample windows at 1/8, 3/8, 5/8 and 7/8 of the range: the sieve's cost grows with q
@@ -768,9 +776,13 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
             ItemTable it;
             rc = build_items(c, (1u << tl) + HALO + WIN_PAD, &it);
             if (rc) return rc;
-            const size_t lds = fused_lds_bytes(1u << tl, it.n_med);
+            ItemTable wt;
+            rc = build_items(c, (1u << (tl - 1)) + WH_HALO + 64, &wt, true);
+            if (rc) return rc;
+            const bool wheel = limit <= WHEEL_N_MAX;
             for (uint32_t wg = 1; wg <= 2; ++wg) {
-                if (lds * wg > LDS_PER_CU) continue;
+                const size_t lds = wheel ? wheel_lds_bytes(wt.n_med, wg) : fused_lds_bytes(1u << tl, it.n_med);
+                if (lds * wg > LDS_PER_CU || (wheel && wg == 2 && (1u << (tl - 1)) > WH_CLASS_HALF)) continue;
                 gb_tuning t = def;
                 t.tile_log2 = tl;
                 t.wg_per_cu = wg;

@@ -805,7 +805,7 @@ __device__ __forceinline__ uint32_t wheel_p2(uint32_t off, uint32_t p, uint32_t 
     return i >= end ? end : (uint32_t)i;
 }
 
-template <int BS>
+template <int BS, uint32_t SW>
 __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict__ med0, int64_t K0, uint32_t nbits,
                                  const Tables& T, uint32_t tid, const PrimeRes* __restrict__ pr, int64_t kbase) {
     const uint32_t nw64 = (nbits + 63) >> 6;
@@ -813,7 +813,7 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         const uint32_t r = s ? 5u : 1u;
-        uint64_t* sw = reinterpret_cast<uint64_t*>(wh + s * WH_STREAM_WORDS);
+        uint64_t* sw = reinterpret_cast<uint64_t*>(wh + s * SW);
         uint32_t j0[NPAT - 1];
 #pragma unroll
         for (int i = 1; i < NPAT; ++i) {
@@ -858,7 +858,7 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
     if (tid >= 1 && tid < NPAT) {
         const uint32_t p = pat_p(tid);
         const int64_t i = (int64_t)(p / 6) - K0;
-        if (i >= 0 && i < (int64_t)nbits) atomicOr(&wh[(p % 6 == 5 ? WH_STREAM_WORDS : 0) + (uint32_t)(i >> 5)], 1u << (i & 31));
+        if (i >= 0 && i < (int64_t)nbits) atomicOr(&wh[(p % 6 == 5 ? SW : 0) + (uint32_t)(i >> 5)], 1u << (i & 31));
     }
     // 3a) wave items: a wavefront takes 64 * WAVE_MARKS marks of one prime in
     //     one stream, lane l the marks l, l + 64, ...  Those sit 64p bits apart:
@@ -871,7 +871,7 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
         for (uint32_t u = __builtin_amdgcn_readfirstlane(tid >> 6); u < n_wave; u += BS / 64) {
             const uint32_t it = T.items[u], p = T.items_p[u];
             const uint32_t sidx = (it >> 12) & 1u;
-            uint32_t* S = wh + sidx * WH_STREAM_WORDS;
+            uint32_t* S = wh + sidx * SW;
             const uint32_t i0 = med0[2 * ((it >> 13) - NPAT) + sidx];
             const uint32_t first = (it & 4095u) * (64 * WAVE_MARKS);   // the item's first mark number
             uint32_t b = i0 + (first + lane) * p;
@@ -900,7 +900,7 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
                 p_n = T.items_p[un];
             }
             const uint32_t sidx = (it >> 12) & 1u;
-            uint32_t* S = wh + sidx * WH_STREAM_WORDS;
+            uint32_t* S = wh + sidx * SW;
             const uint32_t i = med0[2 * ((it >> 13) - NPAT) + sidx] + (it & 4095u) * (CHUNK * p);
             if (i + (CHUNK - 1) * p < nbits)
                 clear_run_n<CHUNK>(S, i, p);
@@ -934,7 +934,7 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
                 o5 = wheel_p2(o5, e.p, 5, K0, nbits);
             }
             clear_run(wh, o1, e.p, nbits);
-            clear_run(wh + WH_STREAM_WORDS, o5, e.p, nbits);
+            clear_run(wh + SW, o5, e.p, nbits);
             k = kn;
         }
     }
@@ -1124,12 +1124,13 @@ __device__ __forceinline__ void class_range(uint64_t n0, uint32_t c, uint32_t le
     if (jhi < jlo) jhi = jlo;
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void wheel_kernel(Tables T, MarkParams mp, const WinDesc* __restrict__ wins,
+template <int BS, uint32_t CMAX>
+__global__ __launch_bounds__(BS, 4) void wheel_kernel(Tables T, MarkParams mp, const WinDesc* __restrict__ wins,
                                                    uint32_t nwin, uint64_t ntiles, uint32_t cls_len,
                                                    uint64_t* __restrict__ tile_counter, SlowList sl,
                                                    StatRec* __restrict__ rec, const PrimeRes* __restrict__ pr) {
-    __shared__ __align__(16) uint32_t wh[2 * WH_STREAM_WORDS];   // [S1 | S5] at fixed LDS addresses
+    constexpr uint32_t SW = wh_stream_words(CMAX);
+    __shared__ __align__(16) uint32_t wh[2 * SW];   // [S1 | S5] at fixed LDS addresses
     extern __shared__ uint32_t lds[];                               // [stats | tile id | medium first marks]
     LaneStats* red = reinterpret_cast<LaneStats*>(lds);
     uint64_t& s_tile = *reinterpret_cast<uint64_t*>(red + BS / 64);
@@ -1138,7 +1139,7 @@ __global__ __launch_bounds__(BS) void wheel_kernel(Tables T, MarkParams mp, cons
     const uint32_t nbits = cls_len + WH_HALO + 64;
     const uint32_t nw = cls_len / 64;
     const uint64_t* s1 = reinterpret_cast<const uint64_t*>(wh);
-    const uint64_t* s5 = reinterpret_cast<const uint64_t*>(wh + WH_STREAM_WORDS);
+    const uint64_t* s5 = reinterpret_cast<const uint64_t*>(wh + SW);
     for (;;) {
         if (tid == 0) s_tile = atomicAdd((unsigned long long*)tile_counter, 1ull);
         __syncthreads();
@@ -1154,7 +1155,7 @@ __global__ __launch_bounds__(BS) void wheel_kernel(Tables T, MarkParams mp, cons
         const uint64_t n_lo = wd.start, n_hi = wd.start + 2 * (wd.evens - 1);
         const uint64_t n0 = (n_lo - n_lo % 6) + 6ull * cls_len * (t - wd.tile0);
         const int64_t K0 = (int64_t)(n0 / 6) - (int64_t)WH_HALO;
-        sieve_wheel_tile<BS>(wh, med0, K0, nbits, T, tid, pr + wd.r0off, wd.kbase);
+        sieve_wheel_tile<BS, SW>(wh, med0, K0, nbits, T, tid, pr + wd.r0off, wd.kbase);
         LaneStats st = lane_stats_zero();
         uint32_t jlo0, jhi0, jlo2, jhi2, jlo4, jhi4;
         class_range(n0, 0, cls_len, n_lo, n_hi, jlo0, jhi0);
@@ -1560,9 +1561,13 @@ hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, co
     return hipGetLastError();
 }
 
-size_t wheel_lds_bytes(uint32_t n_med) {
+size_t wheel_lds_bytes(uint32_t n_med, uint32_t wg_per_cu) {
     const uint32_t med = n_med > NPAT ? n_med - NPAT : 1;
-    return (size_t)2 * WH_STREAM_WORDS * 4 + sizeof(LaneStats) * (FUSED_BS / 64) + 8 + (size_t)2 * med * 4;
+    if (wg_per_cu >= 2)
+        return (size_t)2 * wh_stream_words(WH_CLASS_HALF) * 4 + sizeof(LaneStats) * (FUSED_BS / 128) + 8 +
+               (size_t)2 * med * 4;
+    return (size_t)2 * wh_stream_words(WH_CLASS_MAX) * 4 + sizeof(LaneStats) * (FUSED_BS / 64) + 8 +
+           (size_t)2 * med * 4;
 }
 
 hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin,
@@ -1575,15 +1580,25 @@ hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* 
 }
 
 hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
-                        uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint64_t* tile_counter, SlowList sl,
-                        StatRec* rec, const PrimeRes* pr) {
+                        uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint32_t wg_per_cu, uint64_t* tile_counter,
+                        SlowList sl, StatRec* rec, const PrimeRes* pr) {
     if (cls_len > WH_CLASS_MAX || cls_len < 64 || (cls_len & 63)) return hipErrorInvalidValue;
     const uint32_t med = T.n_med > NPAT ? T.n_med - NPAT : 1;
+    if (wg_per_cu >= 2) {   // two half-size workgroups per CU: one's LDS phase can overlap the other's VALU phase
+        if (cls_len > WH_CLASS_HALF) return hipErrorInvalidValue;
+        constexpr int BS = FUSED_BS / 2;
+        const size_t lds = sizeof(LaneStats) * (BS / 64) + 8 + (size_t)2 * med * 4;
+        hipError_t e = allow_lds(wheel_kernel<BS, WH_CLASS_HALF>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((wheel_kernel<BS, WH_CLASS_HALF>), dim3(grid), dim3(BS), lds, s, T, mp, wins, nwin, ntiles,
+                           cls_len, tile_counter, sl, rec, pr);
+        return hipGetLastError();
+    }
     const size_t lds = sizeof(LaneStats) * (FUSED_BS / 64) + 8 + (size_t)2 * med * 4;
-    hipError_t e = allow_lds(wheel_kernel<FUSED_BS>, lds);
+    hipError_t e = allow_lds(wheel_kernel<FUSED_BS, WH_CLASS_MAX>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(wheel_kernel<FUSED_BS>, dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin, ntiles, cls_len,
-                       tile_counter, sl, rec, pr);
+    hipLaunchKernelGGL((wheel_kernel<FUSED_BS, WH_CLASS_MAX>), dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin,
+                       ntiles, cls_len, tile_counter, sl, rec, pr);
     return hipGetLastError();
 }
 

@@ -102,16 +102,18 @@ hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, co
 // Wheel (mod 6) form of the fused fast path: tiles of 3 * cls_len even n (cls_len per
 // class n mod 6), the window kept as two streams of q = 6k + 1 and q = 6k + 5.
 constexpr uint32_t WH_HALO = 1408;                   // k below n0 / 6 (covers p <= 8191: 1366)
-constexpr uint32_t WH_CLASS_MAX = 1u << 19;          // n per class per tile, at most
-constexpr uint32_t WH_STREAM_WORDS = (WH_CLASS_MAX + WH_HALO + 64) / 32 + 2;   // u32 per stream
-size_t wheel_lds_bytes(uint32_t n_med);
+constexpr uint32_t WH_CLASS_MAX = 1u << 19;          // n per class per tile, at most (one workgroup per CU)
+constexpr uint32_t WH_CLASS_HALF = 1u << 18;         // ... with two half-size workgroups per CU
+// u32 per stream for tiles of at most cmax n per class
+__host__ __device__ constexpr uint32_t wh_stream_words(uint32_t cmax) { return (cmax + WH_HALO + 64) / 32 + 2; }
+size_t wheel_lds_bytes(uint32_t n_med, uint32_t wg_per_cu);
 // pr[wins[w].r0off + k]: prime k as window w's stream S1 sees it (e = first stream bit of a
 // multiple of p, counted from k = kbase).
 hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin,
                                  uint32_t max_np, PrimeRes* pr);
 hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
-                        uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint64_t* tile_counter, SlowList sl,
-                        StatRec* rec, const PrimeRes* pr);
+                        uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint32_t wg_per_cu, uint64_t* tile_counter,
+                        SlowList sl, StatRec* rec, const PrimeRes* pr);
 
 // Resolve slow-list entries: search primes above sp[K-1] up to bound.
 hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,

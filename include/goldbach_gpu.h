@@ -118,9 +118,11 @@ int gb_verify_segment(gb_ctx *ctx, uint64_t seg_start, uint64_t seg_even_count, 
 
 typedef struct gb_tuning {
     uint32_t form;          /* GB_FORM_* */
-    uint32_t tile_log2;     /* batched: evens per tile = 2^tile_log2 (0 = 20) */
+    uint32_t tile_log2;     /* batched: tile size 2^tile_log2 (0: 19 with wg_per_cu 2 when every n is
+                               below 2^44 - 2^24, where tiles use the mod-6 wheel and hold
+                               1.5 * 2^tile_log2 evens; otherwise 20) */
     uint64_t seg_evens;     /* plain: evens per segment (0 = 2^27) */
-    uint32_t wg_per_cu;     /* batched: persistent workgroups per CU (0 = default) */
+    uint32_t wg_per_cu;     /* batched: persistent workgroups per CU (0 = default; 2 needs tile_log2 <= 19) */
     uint32_t flags;         /* GB_WANT_PMIN | GB_TUNE_AUTO */
 } gb_tuning;
 

@@ -72,3 +72,19 @@ def test_windows_around_the_wheel_edge(ctx, orc):
             st, _, _ = orc.verify_segment(a, e, 10**6, want_bits=False, want_pmin=False)
             for k in KEYS:
                 assert got[k] == st[k], (a, e, k)
+
+
+@pytest.mark.parametrize("tile_log2", [12, 16, 19])
+def test_two_workgroups_per_cu(ctx, gb, orc, tile_log2):
+    # half-size workgroups, two per CU (the default below 2^44 is tile_log2 19 this way)
+    cls = 1 << (tile_log2 - 1)
+    for start, limit in ((4, 6 * cls * 3), (1_000_002, 1_000_002 + 6 * cls * 2 + 10)):
+        ref = orc.verify_range(start, limit, 1 << 22, 10**6)
+        got = ctx.verify_range(start, limit, 10**6, form=gb.GB_FORM_BATCHED, tile_log2=tile_log2, wg_per_cu=2,
+                               want_pmin=True)
+        for k in KEYS:
+            assert got[k] == ref[k], (tile_log2, start, k)
+        fast = ctx.verify_range(start, limit, 10**6, form=gb.GB_FORM_BATCHED, tile_log2=tile_log2, wg_per_cu=2)
+        assert (fast["evens"], fast["phase2"], fast["failures"]) == (ref["evens"], 0, 0)
+    with pytest.raises(gb.GoldbachError):
+        ctx.verify_range(4, 10**6, 10**6, form=gb.GB_FORM_BATCHED, tile_log2=20, wg_per_cu=2)
