@@ -3,12 +3,13 @@
 
 One step = one complete verification pass over all 499,999,999,999 even n in
 [4, 1e12] with P_SMALL = 1e6, in the batched persistent form (one fused
-sieve+mark kernel over 2^20-even tiles from a device queue, replayed as a
+sieve+mark kernel, two workgroups per CU, over mod-6 wheel tiles of 3 * 2^18
+even n from a device queue, replayed as a
 hipGraph).  The plain per-segment form (2^27 evens per segment, one host sync
 per segment, as the reference's loop) is timed on the same range beside it.
 
 The JSON line carries:
-  roofline      the fused kernel: algorithmic bytes (0.375 B per even n:
+  roofline      the fused wheel kernel: algorithmic bytes (0.375 B per even n:
                 window bitset written + read, verified bitmap written) over
                 its average launch time, measured with HIP events on the
                 library's stream around each launch; traffic from the
@@ -44,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES_PER_EVEN = 0.375
 REF_PUBLISHED_EVENS_PER_S = 499_999_999_999 / 37.440   # RESULTS.md:48, 1x RTX 5090
 METRIC = "even n verified/s at N=1e12 on 1 GPU, batched vs plain; p99 query latency"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_kernel_1e12_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "wheel_kernel_1e12_traffic.json")
 
 
 def device_sync() -> None:
@@ -261,11 +262,12 @@ def main() -> int:
         "dtype": "u64",
         "data": "synthetic (the input is the integer range itself)",
         "config": {"workload": "all even n in [4, 1e12], batched persistent form", "limit": LIMIT,
-                   "p_small": P_SMALL, "tile_evens": 1 << 20, "parallelism": "1 GPU",
+                   "p_small": P_SMALL, "tile_evens": 3 << 18, "workgroups_per_cu": 2,
+                   "parallelism": "1 GPU",
                    "plain_seg_evens": PLAIN_SEG},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "fused_kernel", "kernel_ms": kms,
+                     "kernel": "wheel_kernel<512, 262144>", "kernel_ms": kms,
                      "alg_bytes_per_launch": ALG_BYTES_PER_EVEN * evens},
         "cpu_baseline": cpu,
         "plain": plain,
