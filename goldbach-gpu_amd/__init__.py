@@ -34,7 +34,7 @@ GB_FORM_BATCHED = 1
 EXPORTS = (
     "gb_open", "gb_close", "gb_last_error", "gb_device_count", "gb_device_info",
     "gb_sieve_window", "gb_verify_segment", "gb_verify_range", "gb_verify_windows",
-    "gb_min_partition_batch", "gb_autotune",
+    "gb_min_partition_batch", "gb_autotune", "gb_count_partitions",
     "gb_serve_open", "gb_serve_submit", "gb_serve_poll", "gb_serve_stats_get", "gb_serve_close",
     "gb_serve_last_error",
 )
@@ -114,6 +114,7 @@ def library() -> ctypes.CDLL:
         "gb_verify_windows": (ctypes.c_int, [vp, ctypes.POINTER(GbWindow), u32, u64, u32,
                                              ctypes.POINTER(GbSegStats), ctypes.POINTER(GbSegStats)]),
         "gb_min_partition_batch": (ctypes.c_int, [vp, vp, u32, vp]),
+        "gb_count_partitions": (ctypes.c_int, [vp, u64, u64, vp]),
         "gb_autotune": (ctypes.c_int, [vp, u64, u64, u64, u32, u32, ctypes.POINTER(GbTuning),
                                        ctypes.POINTER(ctypes.c_double)]),
         "gb_serve_open": (ctypes.c_int, [vp, ctypes.POINTER(GbServeConfig), ctypes.POINTER(vp)]),
@@ -239,6 +240,12 @@ class Context:
         n = np.ascontiguousarray(np.asarray(ns, dtype=np.uint64))
         out = np.zeros(n.shape[0], dtype=np.uint64)
         self._check(self._lib.gb_min_partition_batch(self._h, _ptr(n), n.shape[0], _ptr(out)))
+        return out
+
+    def count_partitions(self, start: int, count: int) -> np.ndarray:
+        """c(n) for the count even n from start: primes p <= n/2 with n - p prime."""
+        out = np.zeros(count, dtype=np.uint64)
+        self._check(self._lib.gb_count_partitions(self._h, start, count, _ptr(out)))
         return out
 
     def server(self, slots: int = 0, chunks_per_step: int = 0, mem_budget: int = 0, queue_cap: int = 0) -> "Server":

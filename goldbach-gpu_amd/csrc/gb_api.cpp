@@ -34,6 +34,7 @@ constexpr uint32_t SLOW_CAP = 1u << 20;
 constexpr uint32_t SEED_LIMIT = 4095;                    // trial-division seed (covers sqrt(2^22))
 constexpr uint32_t SP_TABLE_CAP = 300000;                // odd primes below 2^22: 295,946
 constexpr uint64_t WHEEL_N_MAX = (1ull << 44) - (1ull << 24);   // wheel windows stay where the table sieves q
+constexpr uint64_t COUNT_N_MAX = 1ull << 36;             // partition counts: bitsets of 2^35 bits, 4 GiB each
 
 std::mutex g_open_mu;
 std::string g_open_err;
@@ -896,6 +897,65 @@ int gb_verify_windows(gb_ctx* c, const gb_window* windows, uint32_t count, uint6
     }
     tot.kernel_ms = ms;
     if (total) *total = tot;
+    return GB_OK;
+}
+
+int gb_count_partitions(gb_ctx* c, uint64_t start, uint64_t count, uint64_t* counts_out) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (!counts_out) return fail(c, GB_EINVAL, "counts_out is NULL");
+    if (count == 0) return GB_OK;
+    if (start < 4 || (start & 1)) return fail(c, GB_EINVAL, "start must be even and >= 4");
+    if (start > COUNT_N_MAX || count - 1 > (COUNT_N_MAX - start) / 2) return fail(c, GB_ERANGE, "partition counts reach n <= 2^36 only");
+    HCK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const uint64_t n_end = start + 2 * (count - 1);
+    std::fill(counts_out, counts_out + count, 0ull);
+    if (n_end < 6) {                       // only n = 4: 2 + 2
+        counts_out[0] = 1;
+        return GB_OK;
+    }
+    // P: odd primes q in [3, n_end - 3] (bit a <-> 2a + 3), M = (n_end - 6) / 2 its last index
+    const uint64_t q_high = n_end - 3, M = (q_high - 3) / 2;
+    const uint64_t nbits = M + 1, pwords = (nbits + 63) / 64 + 2;
+    ItemTable it;
+    int rc = build_items(c, PLAIN_TILE_BITS, &it);
+    if (rc) return rc;
+    const Tables T = make_tables(c, it);
+    uint64_t *d_P = nullptr, *d_R = nullptr, *d_cnt = nullptr;
+    auto release = [&]() {
+        dfree(d_P);
+        dfree(d_R);
+        dfree(d_cnt);
+    };
+    if (dalloc(&d_P, pwords) != hipSuccess || dalloc(&d_R, pwords) != hipSuccess ||
+        dalloc(&d_cnt, count) != hipSuccess) {
+        release();
+        return fail(c, GB_ENOMEM, "partition counts: device allocation failed");
+    }
+    rc = ensure_wins(c, 1);
+    if (rc == GB_OK) {
+        WinDesc wd{0, 0, 0, 0, 0, 0, 0, 0, 0};
+        set_residues(c, &wd, 3, q_high, 0);
+        rc = ensure_r0(c, wd.np);
+        if (rc == GB_OK) {
+            hipError_t e = launch_fill(s, d_P, pwords, 0);
+            if (e == hipSuccess) e = launch_fill(s, d_cnt, count, 0);
+            if (e == hipSuccess) e = hipMemcpyAsync(c->d_wins, &wd, sizeof wd, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, 1, wd.np, c->d_r0);
+            if (e == hipSuccess) e = launch_sieve_window(s, T, 3, 0, nbits, PLAIN_TILE_BITS, d_P, 0, c->d_r0);
+            if (e == hipSuccess) e = hipEventRecord(c->ev0, s);
+            if (e == hipSuccess) e = launch_count_partitions(s, d_P, d_R, M, pwords, start, n_end, d_cnt);
+            if (e == hipSuccess) e = hipEventRecord(c->ev1, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(counts_out, d_cnt, count * 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) rc = fail(c, GB_EDEVICE, std::string("partition counts: ") + hipGetErrorString(e));
+        }
+    }
+    release();
+    if (rc) return rc;
+    if (start == 4) counts_out[0] += 1;   // 4 = 2 + 2, the only partition with p = 2
     return GB_OK;
 }
 

@@ -1658,4 +1658,118 @@ hipError_t launch_min_partition(hipStream_t s, const Tables& T, const uint64_t* 
     return hipGetLastError();
 }
 
+// ===========================================================================
+// Goldbach partition counts c(n) = #{p <= n/2 : p and n - p prime} for a
+// range of even n (cpu_goldbach.cpp:29-36, goldbach_count, restated for a
+// range).  With P the odd prime bitset (bit a <-> 2a + 3) and K = (n - 6)/2,
+// c(n) = [n = 4] + #{a <= K/2 : P[a] and P[K - a]}: an anti-diagonal of P
+// against itself.  R[j] = P[M - j] (M the last index) turns it into
+// popcount(P[64w..] & R[64w + M - K ..]) over words w <= K/128.
+//
+// count_kernel: a workgroup takes 1024 even n in a block of 8192 (wave v,
+// lane l: n = base + 2(16g + v) + 128l), so every lane of a wave shifts R by
+// the same amount mod 64 (one funnel per word, no divergence), and a chunk of
+// CP_CHUNK words of P and the matching R words are staged in LDS once for all
+// 1024 n.  Per lane and word: two LDS reads (P broadcast, R sliding), a
+// funnel, an AND and a popcount.
+// ===========================================================================
+namespace {
+constexpr uint32_t CP_CHUNK = 2048;
+}
+
+// R[w] = bit-reverse of P over indices [0, M]: bit j of R is P[M - j]; words past M are zero.
+__global__ void reverse_bits_kernel(const uint64_t* __restrict__ P, uint64_t M, uint64_t nwords,
+                                    uint64_t* __restrict__ R) {
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t out = 0;
+        const uint64_t j0 = 64 * w;
+        if (j0 <= M) {
+            // P bits [M - j0 - 63, M - j0], read as a 64-bit window ending at M - j0
+            const int64_t hi_bit = (int64_t)(M - j0);          // P index of R bit j0
+            const int64_t lo_bit = hi_bit - 63;                // may be negative
+            uint64_t win;
+            if (lo_bit >= 0) {
+                const uint64_t q = (uint64_t)lo_bit >> 6, sh = (uint64_t)lo_bit & 63;
+                win = funnel64(P[q], sh ? P[q + 1] : 0, (uint32_t)sh);
+            } else {
+                win = P[0] << (uint32_t)(-lo_bit);             // bits below index 0 are absent
+            }
+            // win bit t <-> P[lo_bit + t]; R bit (j0 + u) <-> P[hi_bit - u] = win bit (63 - u)
+            const uint32_t lo = (uint32_t)win, hi = (uint32_t)(win >> 32);
+            out = ((uint64_t)__builtin_bitreverse32(lo) << 32) | __builtin_bitreverse32(hi);
+            if (M - j0 < 63) out &= (2ull << (M - j0)) - 1;    // bits j > M are zero
+        }
+        R[w] = out;
+    }
+}
+
+__global__ __launch_bounds__(1024) void count_kernel(const uint64_t* __restrict__ P, const uint64_t* __restrict__ R,
+                                                     uint64_t M, uint64_t pwords, uint64_t start, uint64_t n_end,
+                                                     uint64_t nchunks, uint64_t* __restrict__ counts) {
+    __shared__ uint64_t sP[CP_CHUNK];
+    __shared__ uint64_t sR[CP_CHUNK + 66];
+    const uint64_t b = blockIdx.x >> 2;
+    const uint32_t g = blockIdx.x & 3;
+    const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const uint64_t nb = start + 8192 * b;
+    const uint64_t n = nb + 2 * (16 * g + wv) + 128 * (uint64_t)lane;
+    const bool valid = n <= n_end && n >= 6;
+    const uint64_t K = valid ? (n - 6) / 2 : 0;
+    const uint64_t amax = K / 2;                      // a <= K/2: p <= n/2
+    const uint64_t wl = amax >> 6;                    // this lane's last word
+    const uint64_t last_mask = (amax & 63) == 63 ? ~0ull : ((2ull << (amax & 63)) - 1);
+    // the workgroup's n span: K in [K_lo, K_hi]
+    const uint64_t n_lo = nb + 2 * (16 * g) > 6 ? nb + 2 * (16 * g) : 6;
+    const uint64_t n_hi_raw = nb + 2 * (16 * g + 15) + 128 * 63;
+    const uint64_t n_hi = n_hi_raw < n_end ? n_hi_raw : n_end;
+    if (n_hi < n_lo) return;                          // uniform: nothing to count here
+    const uint64_t K_lo = (n_lo - 6) / 2, K_hi = (n_hi - 6) / 2;
+    const uint64_t s_min = M - K_hi, w_hi = (K_hi / 2) >> 6;
+    const uint64_t s = M - K;
+    const uint32_t sh = (uint32_t)(s & 63);           // the same for every lane of the wave
+    const uint32_t ro = (uint32_t)((s >> 6) - (s_min >> 6));
+    uint64_t acc = 0;
+    (void)K_lo;
+    for (uint64_t chunk = blockIdx.y; chunk < nchunks; chunk += gridDim.y) {
+        const uint64_t w0 = chunk * CP_CHUNK;
+        if (w0 > w_hi) break;                         // uniform
+        const uint64_t rb = w0 + (s_min >> 6);
+        for (uint32_t i = tid; i < CP_CHUNK; i += 1024) sP[i] = w0 + i < pwords ? P[w0 + i] : 0;
+        for (uint32_t i = tid; i < CP_CHUNK + 66; i += 1024) sR[i] = rb + i < pwords ? R[rb + i] : 0;
+        __syncthreads();
+        if (valid && wl >= w0) {
+            const uint64_t wend = (wl - w0 + 1) < CP_CHUNK ? (wl - w0 + 1) : CP_CHUNK;
+            uint64_t r0 = sR[ro];
+            for (uint32_t i = 0; i < (uint32_t)wend; ++i) {
+                const uint64_t r1 = sR[ro + i + 1];
+                uint64_t v = funnel64(r0, r1, sh) & sP[i];
+                if (w0 + i == wl) v &= last_mask;
+                acc += __popcll(v);
+                r0 = r1;
+            }
+        }
+        __syncthreads();
+    }
+    if (valid && acc) atomicAdd((unsigned long long*)&counts[(n - start) / 2], (unsigned long long)acc);
+}
+
+hipError_t launch_count_partitions(hipStream_t s, const uint64_t* P, uint64_t* R, uint64_t M, uint64_t pwords,
+                                   uint64_t start, uint64_t n_end, uint64_t* counts) {
+    uint64_t blocks = (pwords + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(reverse_bits_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, P, M, pwords, R);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint64_t evens = (n_end - start) / 2 + 1;
+    const uint64_t nb = (evens + 4095) / 4096;      // blocks of 8192 in n
+    const uint64_t w_top = (((n_end >= 6 ? n_end - 6 : 0) / 2) / 2 >> 6) + 1;
+    const uint64_t nchunks = (w_top + CP_CHUNK - 1) / CP_CHUNK;
+    if (nb * 4 > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const uint32_t gy = (uint32_t)(nchunks < 65535 ? nchunks : 65535);
+    hipLaunchKernelGGL(count_kernel, dim3((uint32_t)(nb * 4), gy), dim3(1024), 0, s, P, R, M, pwords, start, n_end,
+                       nchunks, counts);
+    return hipGetLastError();
+}
+
 }  // namespace gbk

@@ -133,6 +133,12 @@ hipError_t launch_compact_primes(hipStream_t s, const uint64_t* bits, uint32_t n
                                  uint32_t* primes_out, uint32_t cap, uint32_t* count_out);
 hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, double* inv, uint32_t* c32, uint32_t* d5);
 
+// Partition counts: counts[(n - start)/2] += #{a <= K/2 : P[a] and P[K - a]}, K = (n - 6)/2, for even
+// n in [start, n_end]; P: odd prime bitset over indices [0, M] in pwords words (two zero words past M);
+// R: pwords words of scratch, filled with P reversed.  counts must start zeroed; n = 4's 2 + 2 is not added.
+hipError_t launch_count_partitions(hipStream_t s, const uint64_t* P, uint64_t* R, uint64_t M, uint64_t pwords,
+                                   uint64_t start, uint64_t n_end, uint64_t* counts);
+
 // Serving step records: query n resumes its scan at candidate index k.
 struct ServeIn {
     uint64_t n;

@@ -169,6 +169,16 @@ int gb_verify_windows(gb_ctx *ctx, const gb_window *windows, uint32_t count, uin
  * (single_check.cu:95-282), whose p is the first GPU finder, not the minimum. */
 int gb_min_partition_batch(gb_ctx *ctx, const uint64_t *n, uint32_t count, uint64_t *p_out);
 
+/* ---- partition counts (cpu_goldbach.cpp:29-36, goldbach_count) -------- */
+
+/* counts_out[i] = c(n) for n = start + 2i, i < count: the number of primes
+ * p <= n/2 with n - p prime (unordered partitions; 4 = 2 + 2 counts once).
+ * start even >= 4; every n at most 2^36.  The reference's goldbach_count
+ * scans p from 2 to n/2 over a byte table for one n; here the odd prime
+ * bitset and its reversal are built on the device once and each c(n) is an
+ * AND + popcount along an anti-diagonal, 64 pairs per word. */
+int gb_count_partitions(gb_ctx *ctx, uint64_t start, uint64_t count, uint64_t *counts_out);
+
 /* ---- serving loop with continuous batching ---------------------------- */
 
 /* A serving engine on one context: a host thread that keeps up to `slots`

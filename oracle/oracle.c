@@ -437,3 +437,25 @@ int orc_cpu_goldbach(uint64_t limit, uint64_t *checked, uint64_t *failures, uint
     if (first_fail) *first_fail = ff;
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Partition counts, cpu_goldbach.cpp:29-36 restated over one byte table.      */
+/* ------------------------------------------------------------------------- */
+int orc_count_partitions(uint64_t start, uint64_t count, uint64_t *out, int threads) {
+    if (start < 4 || (start & 1) || count == 0) return -1;
+    const uint64_t n_max = start + 2 * (count - 1);
+    uint8_t *isp = (uint8_t *)malloc(n_max + 1);
+    if (!isp) return -1;
+    if (orc_segmented_sieve(0, n_max, isp) != 0) { free(isp); return -1; }
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < (int64_t)count; ++i) {
+        const uint64_t n = start + 2 * (uint64_t)i;
+        uint64_t c = 0;
+        for (uint64_t p = 2; p <= n / 2; ++p)
+            if (isp[p] && isp[n - p]) ++c;
+        out[i] = c;
+    }
+    free(isp);
+    return 0;
+}

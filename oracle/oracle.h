@@ -23,6 +23,7 @@
  *                           :420-485 (per-segment loop), :160-179 (q resolution)
  *   orc_verify_range        src/goldbach.cu:420-433, 689, 786-801 (segment walk + summary)
  *   orc_cpu_goldbach        src/cpu_goldbach.cpp:45-130
+ *   orc_count_partitions    src/cpu_goldbach.cpp:29-36 (goldbach_count)
  *
  * Parity pins (checked in tests/test_oracle.py):
  *   pi(10^k) k=1..8 and primes <= 30  -- src/test_sieve.cpp:31-40, src/test_bitset.cpp:17-50
@@ -78,6 +79,12 @@ uint64_t orc_first_strong_pseudoprime(int k, uint64_t limit);
 /* Smallest prime p <= min(bound, n/2) with n-p prime; 0 if none (or n odd / n < 4). */
 uint64_t orc_min_partition(uint64_t n, uint64_t bound);
 void orc_min_partition_batch(const uint64_t *n, size_t count, uint64_t bound, uint64_t *p_out, int threads);
+
+/* Goldbach partition counts: out[i] = #{p prime, p <= n/2, n - p prime} for n = start + 2i,
+ * i < count (cpu_goldbach.cpp:29-36 goldbach_count: p from 2 to n/2 over a byte table).
+ * One sieve of [0, n_max] then a scan per n (OpenMP over n).  Returns 0, or -1 when start is
+ * odd or below 4 or the table cannot be allocated. */
+int orc_count_partitions(uint64_t start, uint64_t count, uint64_t *out, int threads);
 
 /*
  * One segment of `count` consecutive even n starting at even A >= 4.

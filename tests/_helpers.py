@@ -56,6 +56,8 @@ class Oracle:
         L.orc_window_bits.restype = ctypes.c_int; L.orc_window_bits.argtypes = [u64, u64, vp]
         L.orc_is_prime.restype = ctypes.c_int; L.orc_is_prime.argtypes = [u64]
         L.orc_strong_first.restype = ctypes.c_int; L.orc_strong_first.argtypes = [u64, ctypes.c_int]
+        L.orc_count_partitions.restype = ctypes.c_int
+        L.orc_count_partitions.argtypes = [u64, u64, ctypes.c_void_p, ctypes.c_int]
         L.orc_first_strong_pseudoprime.restype = u64
         L.orc_first_strong_pseudoprime.argtypes = [ctypes.c_int, u64]
         L.orc_min_partition.restype = u64; L.orc_min_partition.argtypes = [u64, u64]
@@ -97,6 +99,11 @@ class Oracle:
 
     def first_strong_pseudoprime(self, k, limit):
         return int(self.L.orc_first_strong_pseudoprime(k, limit))
+
+    def count_partitions(self, start, count, threads=0):
+        out = np.zeros(count, dtype=np.uint64)
+        assert self.L.orc_count_partitions(start, count, out.ctypes.data, threads) == 0
+        return out
 
     def is_prime(self, n):
         return bool(self.L.orc_is_prime(n))
@@ -143,6 +150,7 @@ class Ref:
         L.ref_simple_sieve.restype = ctypes.c_size_t; L.ref_simple_sieve.argtypes = [u64, vp, ctypes.c_size_t]
         L.ref_segmented_sieve.restype = ctypes.c_int; L.ref_segmented_sieve.argtypes = [u64, u64, vp]
         L.ref_prime_bitset.restype = u64; L.ref_prime_bitset.argtypes = [u64, vp, u64, ctypes.c_int]
+        L.ref_goldbach_count.restype = ctypes.c_int; L.ref_goldbach_count.argtypes = [u64, u64, vp]
         self.L = L
 
     def simple_sieve(self, limit):
@@ -161,6 +169,11 @@ class Ref:
         out = np.zeros(max(nw, 1), dtype=np.uint64)
         self.L.ref_prime_bitset(limit, out.ctypes.data, nw, threads)
         return out[:nw]
+
+    def goldbach_count(self, start, count):
+        out = np.zeros(count, dtype=np.uint64)
+        self.L.ref_goldbach_count(start, count, out.ctypes.data)
+        return out
 
 
 def unpack(words, nbits):

@@ -263,3 +263,48 @@ def test_reference_known_answer_programs():
             pytest.skip("oracle/_ref not built")
         r = subprocess.run([exe], capture_output=True, text=True)
         assert r.returncode == 0 and "All tests passed." in r.stdout
+
+
+# ---- partition counts c(n) (cpu_goldbach.cpp:29-36) ----
+
+# Published numbers of Goldbach partitions of 10^k, k = 1..8 (OEIS A065577).
+C_POW10 = (2, 6, 28, 127, 810, 5402, 38807, 291400)
+
+
+def test_count_partitions_literature(orc):
+    for k, want in enumerate(C_POW10, start=1):
+        assert int(orc.count_partitions(10**k, 1)[0]) == want, k
+
+
+def test_count_partitions_brute_force(orc):
+    # tiny n by trial division, independent of every sieve here
+    def isp(m):
+        return m >= 2 and all(m % d for d in range(2, int(m**0.5) + 1))
+    got = orc.count_partitions(4, 500)
+    for i, c in enumerate(got):
+        n = 4 + 2 * i
+        assert int(c) == sum(1 for p in range(2, n // 2 + 1) if isp(p) and isp(n - p)), n
+    # 4 = 2 + 2 and 6 = 3 + 3 are the only one-partition n below 12
+    assert list(got[:4]) == [1, 1, 1, 2]
+
+
+def test_count_partitions_matches_reference(orc, ref):
+    for start, count in ((4, 3000), (999_000, 400), (10**7 - 200, 101)):
+        assert np.array_equal(orc.count_partitions(start, count), ref.goldbach_count(start, count)), start
+
+
+def test_count_partitions_positive_and_parity_sum(orc):
+    # Every even n in [4, N] has a partition, and the sum of c(n) over them is the
+    # number of prime pairs p <= q with p + q even and at most N (a double loop).
+    N = 20_000
+    got = orc.count_partitions(4, (N - 4) // 2 + 1)
+    assert (got > 0).all()
+    pr = [p for p in range(2, N) if all(p % d for d in range(2, int(p**0.5) + 1))]
+    pairs = 0
+    for i, p in enumerate(pr):
+        for q in pr[i:]:
+            if p + q > N:
+                break
+            if (p + q) % 2 == 0:
+                pairs += 1
+    assert int(got.sum()) == pairs
