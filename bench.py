@@ -19,6 +19,9 @@ The JSON line carries:
                 host over 32 sampled 2^24-even segments spread over [4, 1e12].
   high_windows  64 windows above 1e15 (q by pre-sieve + Miller-Rabin) in one
                 persistent launch, evens/s.
+  partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
+                candidate pairs/s (the reference loop's p = 2..n/2 per n), with
+                the oracle's c(n) on 512 n below 3e7 as its CPU baseline.
   serving       the continuous-batching server (gb_serve_*): p50/p99 latency
                 from scheduled arrival to result under open-loop Poisson
                 arrivals at 1e4, 1e5 and 1e6 queries/s, and queries/s for a
@@ -184,6 +187,48 @@ def windows_leg(ctx) -> dict:
             "kernel_ms": tot["kernel_ms"], "phase2": tot["phase2"]}
 
 
+COUNT_LIMIT = 3 * 10**7
+
+
+def count_pairs(start: int, count: int) -> int:
+    """Candidate pairs the reference's goldbach_count loop visits: p = 2..n/2 for each n."""
+    n = start + 2 * np.arange(count, dtype=np.int64)
+    return int((n // 2 - 1).sum())
+
+
+def counts_leg(ctx) -> dict:
+    """Partition counts c(n): every even n in [4, 3e7], and 4096 consecutive n from 1e10."""
+    ctx.count_partitions(4, 10**4)                       # warm
+    cnt = (COUNT_LIMIT - 4) // 2 + 1
+    device_sync()
+    t0 = time.perf_counter()
+    c = ctx.count_partitions(4, cnt)
+    dt = time.perf_counter() - t0
+    assert int(c[(10**7 - 4) // 2]) == 38807 and int(c.min()) >= 1     # c(1e7), OEIS A065577
+    t1 = time.perf_counter()
+    hi = ctx.count_partitions(10**10, 4096)
+    dt_hi = time.perf_counter() - t1
+    assert int(hi[0]) == 18200488                                      # c(1e10)
+    pairs = count_pairs(4, cnt)
+    return {"workload": "c(n) for every even n in [4, 3e7]", "n": cnt, "seconds": dt,
+            "value": pairs / dt, "unit": "pairs/s", "n_per_s": cnt / dt,
+            "near_1e10": {"n": 4096, "seconds": dt_hi, "pairs_per_s": count_pairs(10**10, 4096) / dt_hi}}
+
+
+def cpu_counts_baseline() -> dict:
+    """The oracle's c(n) (reference loop: byte table, p ascending) on 512 n just below 3e7."""
+    from _helpers import Oracle
+
+    orc = Oracle()
+    threads = min(16, os.cpu_count() or 1)
+    start = COUNT_LIMIT - 2 * 511
+    t0 = time.perf_counter()
+    orc.count_partitions(start, 512, threads)
+    dt = time.perf_counter() - t0
+    return {"value": count_pairs(start, 512) / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"c(n) for the 512 even n below 3e7 (sieve included), {threads} threads, {dt:.2f} s wall"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +238,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-windows", action="store_true", help="skip the high-window (Miller-Rabin) leg")
+    ap.add_argument("--no-counts", action="store_true", help="skip the partition-count leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -245,8 +291,11 @@ def main() -> int:
 
     serve = None if args.no_serve else serving(ctx)
     high = None if args.no_windows else windows_leg(ctx)
+    counts = None if args.no_counts else counts_leg(ctx)
     ctx.close()
     cpu = None if args.no_cpu else cpu_baseline()
+    if counts is not None and not args.no_cpu:
+        counts["cpu_baseline"] = cpu_counts_baseline()
 
     line = {
         "metric": METRIC,
@@ -274,6 +323,7 @@ def main() -> int:
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "serving": serve,
         "high_windows": high,
+        "partition_counts": counts,
     }
     print(json.dumps(line))
     return 0

@@ -1719,18 +1719,20 @@ __global__ __launch_bounds__(1024) void count_kernel(const uint64_t* __restrict_
     const uint64_t amax = K / 2;                      // a <= K/2: p <= n/2
     const uint64_t wl = amax >> 6;                    // this lane's last word
     const uint64_t last_mask = (amax & 63) == 63 ? ~0ull : ((2ull << (amax & 63)) - 1);
-    // the workgroup's n span: K in [K_lo, K_hi]
+    // the workgroup's n span, n_lo..n_hi (K up to K_hi)
     const uint64_t n_lo = nb + 2 * (16 * g) > 6 ? nb + 2 * (16 * g) : 6;
     const uint64_t n_hi_raw = nb + 2 * (16 * g + 15) + 128 * 63;
     const uint64_t n_hi = n_hi_raw < n_end ? n_hi_raw : n_end;
     if (n_hi < n_lo) return;                          // uniform: nothing to count here
-    const uint64_t K_lo = (n_lo - 6) / 2, K_hi = (n_hi - 6) / 2;
+    const uint64_t K_hi = (n_hi - 6) / 2;
     const uint64_t s_min = M - K_hi, w_hi = (K_hi / 2) >> 6;
     const uint64_t s = M - K;
-    const uint32_t sh = (uint32_t)(s & 63);           // the same for every lane of the wave
+    // lanes of a wave are 64 apart in K, so s mod 64 is the wave's (taken from lane 0's
+    // nominal n, which may be 4: K = -1 there, still right mod 64)
+    const int64_t n_w = (int64_t)(nb + 2 * (16 * g + wv));
+    const uint32_t sh = __builtin_amdgcn_readfirstlane((uint32_t)(((int64_t)M - (n_w - 6) / 2) & 63));
     const uint32_t ro = (uint32_t)((s >> 6) - (s_min >> 6));
-    uint64_t acc = 0;
-    (void)K_lo;
+    uint32_t acc = 0;                                 // c(n) < 2^32 for n <= 2^36
     for (uint64_t chunk = blockIdx.y; chunk < nchunks; chunk += gridDim.y) {
         const uint64_t w0 = chunk * CP_CHUNK;
         if (w0 > w_hi) break;                         // uniform
@@ -1739,14 +1741,27 @@ __global__ __launch_bounds__(1024) void count_kernel(const uint64_t* __restrict_
         for (uint32_t i = tid; i < CP_CHUNK + 66; i += 1024) sR[i] = rb + i < pwords ? R[rb + i] : 0;
         __syncthreads();
         if (valid && wl >= w0) {
-            const uint64_t wend = (wl - w0 + 1) < CP_CHUNK ? (wl - w0 + 1) : CP_CHUNK;
-            uint64_t r0 = sR[ro];
-            for (uint32_t i = 0; i < (uint32_t)wend; ++i) {
-                const uint64_t r1 = sR[ro + i + 1];
-                uint64_t v = funnel64(r0, r1, sh) & sP[i];
-                if (w0 + i == wl) v &= last_mask;
-                acc += __popcll(v);
-                r0 = r1;
+            // full words first, then the lane's last word (a <= amax) under its mask
+            // R read as 32-bit words so the wave-uniform shift stays below 32: the lane's
+            // R bits for word i start at bit 64(ro + i) + sh of the staged R
+            const bool ends_here = wl - w0 < CP_CHUNK;
+            const uint32_t full = ends_here ? (uint32_t)(wl - w0) : CP_CHUNK;
+            const uint32_t* sR32 = reinterpret_cast<const uint32_t*>(sR);
+            const uint32_t bi = 2 * ro + (sh >> 5), s5 = sh & 31;
+            uint32_t x0 = sR32[bi];
+            uint32_t i = 0;
+            for (; i < full; ++i) {
+                const uint32_t x1 = sR32[bi + 2 * i + 1], x2 = sR32[bi + 2 * i + 2];
+                const uint64_t pw = sP[i];
+                acc += __builtin_popcount(__builtin_amdgcn_alignbit(x1, x0, s5) & (uint32_t)pw) +
+                       __builtin_popcount(__builtin_amdgcn_alignbit(x2, x1, s5) & (uint32_t)(pw >> 32));
+                x0 = x2;
+            }
+            if (ends_here) {
+                const uint32_t x1 = sR32[bi + 2 * i + 1], x2 = sR32[bi + 2 * i + 2];
+                const uint64_t pw = sP[i] & last_mask;
+                acc += __builtin_popcount(__builtin_amdgcn_alignbit(x1, x0, s5) & (uint32_t)pw) +
+                       __builtin_popcount(__builtin_amdgcn_alignbit(x2, x1, s5) & (uint32_t)(pw >> 32));
             }
         }
         __syncthreads();

@@ -9,7 +9,10 @@
 // launch; plain: one launch sequence and host sync per segment, as the
 // reference), --tile-log2=N (batched tile of 2^N evens), --phase2-limit=N
 // (Phase-2 prime bound, 1e8 as in the reference; a test hook), --autotune
-// (time tile / segment-size candidates on samples of the range first).
+// (time tile / segment-size candidates on samples of the range first),
+// --counts (print c(n), the number of partitions p + q = n with p <= q, for
+// every even n of the range, as the reference's cpu_goldbach.cpp:101-103 does
+// with PRINT_COUNTS, then the smallest c(n) seen; n at most 2^36).
 // --batch-size and --async are accepted and have no effect: the device keeps
 // every candidate prime resident, so there are no prime batches to size.
 #include <atomic>
@@ -41,9 +44,10 @@ void print_usage(const char* prog) {
               << "  --start=N        Starting number for verification (default: 4)\n"
               << "  --progress       Show real-time progress updates\n"
               << "  --form=F         batched (default) or plain per-segment launches\n"
-              << "  --tile-log2=N    Batched tile of 2^N evens (12..20, default 20)\n"
+              << "  --tile-log2=N    Batched tile of 2^N evens (12..20; default 19 below 2^44, else 20)\n"
               << "  --phase2-limit=N Phase-2 prime bound (default: 100000000)\n"
               << "  --autotune       Pick tile / segment size by timing samples of the range\n"
+              << "  --counts         Print c(n), the number of Goldbach partitions, for every n\n"
               << "  -h, --help       Show this help message\n";
 }
 
@@ -64,6 +68,7 @@ int main(int argc, char** argv) {
     int requested_gpus = 1;
     bool progress = false;
     bool autotune = false;
+    bool counts = false;
     uint32_t form = GB_FORM_BATCHED;
     std::vector<std::string> positional;
 
@@ -78,6 +83,7 @@ int main(int argc, char** argv) {
             if (a == "--async") continue;
             if (a == "--progress") { progress = true; continue; }
             if (a == "--autotune") { autotune = true; continue; }
+            if (a == "--counts") { counts = true; continue; }
             if (has("--batch-size=")) { (void)std::stoull(a.substr(13)); continue; }
             if (has("--gpus=")) { requested_gpus = std::stoi(a.substr(7)); continue; }
             if (has("--seg-size=")) { SEG_SIZE = std::stoull(a.substr(11)); continue; }
@@ -205,6 +211,41 @@ int main(int argc, char** argv) {
                 if (running.load()) print_progress(processed);
             }
         });
+    }
+
+    if (counts) {
+        // c(n) in slices of 2^22 n: 32 MiB of host buffer whatever the range
+        std::vector<uint64_t> c(std::min<uint64_t>(total_even, 1ull << 22));
+        uint64_t c_min = ~0ull, n_min = 0;
+        auto stop_progress = [&] {
+            if (!progress) return;
+            running.store(false);
+            progress_thread.join();
+            std::cout << "\n";
+        };
+        for (uint64_t done = 0; done < total_even;) {
+            const uint64_t k = std::min<uint64_t>(c.size(), total_even - done);
+            const uint64_t n0 = START + 2 * done;
+            if (gb_count_partitions(ctx, n0, k, c.data()) != GB_OK) {
+                stop_progress();
+                std::cerr << "[!] FATAL ERROR in GPU 0 Worker: " << gb_last_error(ctx) << "\n";
+                gb_close(ctx);
+                return 1;
+            }
+            for (uint64_t i = 0; i < k; ++i) {
+                std::cout << "c(" << n0 + 2 * i << ") = " << c[i] << "\n";
+                if (c[i] < c_min) { c_min = c[i]; n_min = n0 + 2 * i; }
+            }
+            done += k;
+            processed = done;
+        }
+        stop_progress();
+        gb_close(ctx);
+        std::cout << "\n--- Partition Counts Complete ---\n";
+        std::cout << "Even numbers counted   : " << total_even << "\n";
+        std::cout << "Smallest c(n)          : " << c_min << " at n = " << n_min << "\n";
+        std::cout << "Total computation time : " << secs_since(t_main) << " seconds\n";
+        return c_min == 0 ? 1 : 0;
     }
 
     gb_tuning tun{};

@@ -60,3 +60,14 @@ def test_forced_failure_reports_n():
 def test_single_check():
     rc, out = run([SINGLE, "1000000000000"])
     assert rc == 0 and "Goldbach holds. ✓" in out and "1000000000000 = 11 + 999999999989" in out
+
+
+def test_counts_flag(orc):
+    # c(n) lines in the reference's PRINT_COUNTS format (cpu_goldbach.cpp:101-103)
+    rc, out = run([GOLDBACH, "10000", "--start=9000", "--counts"])
+    assert rc == 0, out
+    ref = orc.count_partitions(9000, 501)
+    lines = [l for l in out.splitlines() if l.startswith("c(")]
+    assert lines == [f"c({9000 + 2 * i}) = {int(c)}" for i, c in enumerate(ref)]
+    i = int(ref.argmin())
+    assert f"Smallest c(n)          : {int(ref[i])} at n = {9000 + 2 * i}" in out
