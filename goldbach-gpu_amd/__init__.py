@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libgoldbach_gpu.so")
 HOST_LIB_PATH = os.path.join(PKG_DIR, "lib", "libgoldbach.so")
 BIN_DIR = os.path.join(PKG_DIR, "bin")
 
-GB_OK, GB_EINVAL, GB_ENOMEM, GB_EDEVICE, GB_ENODEV, GB_ERANGE, GB_EAGAIN = range(7)
+GB_OK, GB_EINVAL, GB_ENOMEM, GB_EDEVICE, GB_ENODEV, GB_ERANGE, GB_EAGAIN, GB_EIO = range(8)
 GB_WANT_PMIN = 0x1
 GB_TUNE_AUTO = 0x2
 GB_FORM_PLAIN = 0
@@ -34,7 +34,7 @@ GB_FORM_BATCHED = 1
 EXPORTS = (
     "gb_open", "gb_close", "gb_last_error", "gb_device_count", "gb_device_info",
     "gb_sieve_window", "gb_verify_segment", "gb_verify_range", "gb_verify_windows",
-    "gb_min_partition_batch", "gb_autotune", "gb_count_partitions",
+    "gb_min_partition_batch", "gb_autotune", "gb_count_partitions", "gb_verify_range_logged", "gb_log_read",
     "gb_serve_open", "gb_serve_submit", "gb_serve_poll", "gb_serve_stats_get", "gb_serve_close",
     "gb_serve_last_error",
 )
@@ -76,6 +76,30 @@ class GbServeStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class GbLogHeader(ctypes.Structure):
+    _fields_ = [("start", u64), ("limit", u64), ("p_small", u64), ("phase2_limit", u64), ("chunk_evens", u64),
+                ("want_pmin", u32), ("reserved", u32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class GbLogRecord(ctypes.Structure):
+    _fields_ = [(name, u64) for name in ("chunk", "first_n", "last_n", "evens", "phase2", "failures",
+                                          "first_fail_n", "sum_pmin", "max_pmin", "argmax_n")]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GbLogInfo(ctypes.Structure):
+    _fields_ = [("resumed_chunks", u64), ("resumed_evens", u64), ("done_chunks", u64), ("total_chunks", u64),
+                ("complete", u32), ("truncated", u32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # gb_result as a numpy record (48 bytes, same field order and widths)
 RESULT_DTYPE = np.dtype([("id", "<u8"), ("n", "<u8"), ("p", "<u8"), ("submit_ns", "<u8"), ("done_ns", "<u8"),
                          ("steps", "<u4"), ("reserved", "<u4")])
@@ -115,6 +139,10 @@ def library() -> ctypes.CDLL:
                                              ctypes.POINTER(GbSegStats), ctypes.POINTER(GbSegStats)]),
         "gb_min_partition_batch": (ctypes.c_int, [vp, vp, u32, vp]),
         "gb_count_partitions": (ctypes.c_int, [vp, u64, u64, vp]),
+        "gb_verify_range_logged": (ctypes.c_int, [vp, u64, u64, u64, ctypes.POINTER(GbTuning), ctypes.c_char_p, u64,
+                                                  u64, ctypes.POINTER(GbSegStats), ctypes.POINTER(GbLogInfo), vp]),
+        "gb_log_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(GbLogHeader), ctypes.POINTER(GbLogRecord),
+                                       u64, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
         "gb_autotune": (ctypes.c_int, [vp, u64, u64, u64, u32, u32, ctypes.POINTER(GbTuning),
                                        ctypes.POINTER(ctypes.c_double)]),
         "gb_serve_open": (ctypes.c_int, [vp, ctypes.POINTER(GbServeConfig), ctypes.POINTER(vp)]),
@@ -130,6 +158,22 @@ def library() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def read_log(path: str) -> tuple:
+    """(header dict, list of record dicts, torn) of a run log; needs no device."""
+    lib = library()
+    hdr = GbLogHeader()
+    n = u64(0)
+    torn = u32(0)
+    rc = lib.gb_log_read(os.fsencode(path), ctypes.byref(hdr), None, 0, ctypes.byref(n), ctypes.byref(torn))
+    if rc != GB_OK:
+        raise GoldbachError(rc, f"cannot read run log {path}")
+    recs = (GbLogRecord * max(n.value, 1))()
+    rc = lib.gb_log_read(os.fsencode(path), ctypes.byref(hdr), recs, n.value, ctypes.byref(n), ctypes.byref(torn))
+    if rc != GB_OK:
+        raise GoldbachError(rc, f"cannot read run log {path}")
+    return hdr.as_dict(), [recs[i].as_dict() for i in range(n.value)], bool(torn.value)
 
 
 def device_count() -> int:
@@ -216,6 +260,18 @@ class Context:
         self._check(self._lib.gb_verify_range(self._h, start, limit, p_small, ctypes.byref(tun), ctypes.byref(st),
                                               None))
         return st.as_dict()
+
+    def verify_range_logged(self, start: int, limit: int, p_small: int, log_path: str, chunk_evens: int = 0,
+                            max_chunks: int = 0, form: int = GB_FORM_BATCHED, tile_log2: int = 0,
+                            seg_evens: int = 0, wg_per_cu: int = 0, want_pmin: bool = False) -> tuple:
+        """(stats over every chunk in the log, log info): resumes from log_path where it stopped."""
+        tun = GbTuning(form, tile_log2, seg_evens, wg_per_cu, GB_WANT_PMIN if want_pmin else 0)
+        st = GbSegStats()
+        info = GbLogInfo()
+        self._check(self._lib.gb_verify_range_logged(self._h, start, limit, p_small, ctypes.byref(tun),
+                                                     os.fsencode(log_path), chunk_evens, max_chunks,
+                                                     ctypes.byref(st), ctypes.byref(info), None))
+        return st.as_dict(), info.as_dict()
 
     def autotune(self, start: int, limit: int, p_small: int, form: int = GB_FORM_BATCHED,
                  want_pmin: bool = False) -> tuple:

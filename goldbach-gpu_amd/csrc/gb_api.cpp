@@ -129,6 +129,15 @@ Tables make_tables(const gb_ctx* c, const ItemTable& it) {
     return T;
 }
 
+int normalise_range(gb_ctx* c, uint64_t* start, uint64_t* limit) {
+    if (*limit & 1) --*limit;           // goldbach.cu:594
+    if (*start < 4) *start = 4;         // goldbach.cu:604-605
+    if (*start & 1) ++*start;
+    if (*limit < 4) return fail(c, GB_EINVAL, "LIMIT must be >= 4");
+    if (*start > *limit) return fail(c, GB_EINVAL, "START must not exceed LIMIT");
+    return GB_OK;
+}
+
 }  // namespace gbh
 
 namespace {
@@ -725,14 +734,6 @@ static int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, co
     return GB_OK;
 }
 
-static int normalise_range(gb_ctx* c, uint64_t* start, uint64_t* limit) {
-    if (*limit & 1) --*limit;           // goldbach.cu:594
-    if (*start < 4) *start = 4;         // goldbach.cu:604-605
-    if (*start & 1) ++*start;
-    if (*limit < 4) return fail(c, GB_EINVAL, "LIMIT must be >= 4");
-    if (*start > *limit) return fail(c, GB_EINVAL, "START must not exceed LIMIT");
-    return GB_OK;
-}
 
 int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uint32_t form, uint32_t flags,
                 gb_tuning* best, double* evens_per_s) {

@@ -92,6 +92,10 @@ namespace gbh {
 // Records msg as the context's last error and returns code.
 int fail(gb_ctx* c, int code, const std::string& msg);
 
+// The reference's range adjustments (goldbach.cu:594,604-609): limit down to even, start up to
+// even >= 4; GB_EINVAL when the range is empty.
+int normalise_range(gb_ctx* c, uint64_t* start, uint64_t* limit);
+
 #define HCK(ctx, call)                                                                         \
     do {                                                                                       \
         hipError_t e_ = (call);                                                                \

@@ -12,7 +12,10 @@
 // (time tile / segment-size candidates on samples of the range first),
 // --counts (print c(n), the number of partitions p + q = n with p <= q, for
 // every even n of the range, as the reference's cpu_goldbach.cpp:101-103 does
-// with PRINT_COUNTS, then the smallest c(n) seen; n at most 2^36).
+// with PRINT_COUNTS, then the smallest c(n) seen; n at most 2^36),
+// --log=FILE [--log-chunk=N] (resumable run: every N evens finished append a
+// checksummed record to FILE; running the same command again continues after
+// the last record instead of restarting by hand with --start).
 // --batch-size and --async are accepted and have no effect: the device keeps
 // every candidate prime resident, so there are no prime batches to size.
 #include <atomic>
@@ -48,6 +51,8 @@ void print_usage(const char* prog) {
               << "  --phase2-limit=N Phase-2 prime bound (default: 100000000)\n"
               << "  --autotune       Pick tile / segment size by timing samples of the range\n"
               << "  --counts         Print c(n), the number of Goldbach partitions, for every n\n"
+              << "  --log=FILE       Resumable run: record finished chunks in FILE, continue after them\n"
+              << "  --log-chunk=N    Even integers per run-log record (default: 2^36)\n"
               << "  -h, --help       Show this help message\n";
 }
 
@@ -69,6 +74,8 @@ int main(int argc, char** argv) {
     bool progress = false;
     bool autotune = false;
     bool counts = false;
+    std::string log_path;
+    uint64_t log_chunk = 0;
     uint32_t form = GB_FORM_BATCHED;
     std::vector<std::string> positional;
 
@@ -84,6 +91,8 @@ int main(int argc, char** argv) {
             if (a == "--progress") { progress = true; continue; }
             if (a == "--autotune") { autotune = true; continue; }
             if (a == "--counts") { counts = true; continue; }
+            if (has("--log=")) { log_path = a.substr(6); continue; }
+            if (has("--log-chunk=")) { log_chunk = std::stoull(a.substr(12)); continue; }
             if (has("--batch-size=")) { (void)std::stoull(a.substr(13)); continue; }
             if (has("--gpus=")) { requested_gpus = std::stoi(a.substr(7)); continue; }
             if (has("--seg-size=")) { SEG_SIZE = std::stoull(a.substr(11)); continue; }
@@ -268,7 +277,11 @@ int main(int argc, char** argv) {
                   << " evens/s on the samples)\n\n";
     }
     gb_seg_stats st{};
-    const int rc = gb_verify_range(ctx, START, LIMIT, P_SMALL, &tun, &st, &processed);
+    gb_log_info li{};
+    const int rc = log_path.empty()
+                       ? gb_verify_range(ctx, START, LIMIT, P_SMALL, &tun, &st, &processed)
+                       : gb_verify_range_logged(ctx, START, LIMIT, P_SMALL, &tun, log_path.c_str(), log_chunk, 0, &st,
+                                                &li, &processed);
     const double total_ms = secs_since(t_main) * 1000.0;
     if (progress) {
         running.store(false);
@@ -283,6 +296,10 @@ int main(int argc, char** argv) {
         return 1;
     }
     gb_close(ctx);
+    if (!log_path.empty())
+        std::cout << "Run log        : " << log_path << " (" << li.resumed_chunks << " of " << li.total_chunks
+                  << " chunks resumed, " << li.resumed_evens << " evens" << (li.truncated ? "; damaged tail dropped" : "")
+                  << ")\n";
     if (st.failures) {
         std::cout << "\n[!] Goldbach FAILED at n = " << st.first_fail_n << "\n";
         return 1;

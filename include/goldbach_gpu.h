@@ -47,7 +47,8 @@ typedef enum gb_status {
     GB_EDEVICE = 3,  /* HIP runtime error (the reference throws from CUDA_CHECK, goldbach.cu:105-114) */
     GB_ENODEV = 4,   /* no usable device (goldbach.cu:624-626) */
     GB_ERANGE = 5,   /* request beyond a capacity fixed at gb_open */
-    GB_EAGAIN = 6    /* serving: no result ready / no free slot yet */
+    GB_EAGAIN = 6,   /* serving: no result ready / no free slot yet */
+    GB_EIO = 7       /* run log: file cannot be read or written */
 } gb_status;
 
 typedef struct gb_ctx gb_ctx;
@@ -168,6 +169,48 @@ int gb_verify_windows(gb_ctx *ctx, const gb_window *windows, uint32_t count, uin
  * up to phase2_limit; 0 if none.  Replaces check_single + goldbach_single_kernel
  * (single_check.cu:95-282), whose p is the first GPU finder, not the minimum. */
 int gb_min_partition_batch(gb_ctx *ctx, const uint64_t *n, uint32_t count, uint64_t *p_out);
+
+/* ---- resumable runs (generalises --start, goldbach.cu:581,604-609,689) -- */
+
+/* The reference resumes a long run by restarting it with --start=N past
+ * what the user believes was verified.  A run log records it instead: the
+ * range is cut into chunks of chunk_evens even n (0 = 2^36) and every
+ * finished chunk appends one CRC-32-checked text line (fsync'd before the
+ * next chunk starts).  Calling again with the same log skips the chunks it
+ * holds; a torn or corrupt tail is dropped and recomputed; a log written for
+ * another (start, limit, p_small, phase2_limit, chunk_evens, GB_WANT_PMIN)
+ * is refused with GB_EINVAL.  A chunk with a failure ends the run, as the
+ * reference stops at its first failure (goldbach.cu:476-480).
+ * max_chunks > 0 stops after that many new chunks (a time-sliced job);
+ * *out covers every chunk in the log, its kernel_ms only this call's. */
+typedef struct gb_log_header {
+    uint64_t start, limit, p_small, phase2_limit, chunk_evens;
+    uint32_t want_pmin, reserved;
+} gb_log_header;
+
+typedef struct gb_log_record {
+    uint64_t chunk, first_n, last_n, evens, phase2, failures, first_fail_n, sum_pmin, max_pmin, argmax_n;
+} gb_log_record;
+
+typedef struct gb_log_info {
+    uint64_t resumed_chunks;   /* chunks the log already held */
+    uint64_t resumed_evens;
+    uint64_t done_chunks;      /* chunks in the log after the call */
+    uint64_t total_chunks;
+    uint32_t complete;         /* the log covers the range, or a failure ended the run */
+    uint32_t truncated;        /* a torn or corrupt tail was dropped */
+} gb_log_info;
+
+int gb_verify_range_logged(gb_ctx *ctx, uint64_t start, uint64_t limit, uint64_t p_small, const gb_tuning *tuning,
+                           const char *log_path, uint64_t chunk_evens, uint64_t max_chunks, gb_seg_stats *out,
+                           gb_log_info *info, volatile uint64_t *progress_evens);
+
+/* Reads a run log without a device: its header and the intact, in-order prefix
+ * of records (*count of them; the first cap copied to records, which may be
+ * NULL).  *torn (nullable) = 1 when bytes follow that prefix.  GB_EIO when the
+ * file is missing or empty, GB_EINVAL when its header is damaged. */
+int gb_log_read(const char *path, gb_log_header *header, gb_log_record *records, uint64_t cap, uint64_t *count,
+                uint32_t *torn);
 
 /* ---- partition counts (cpu_goldbach.cpp:29-36, goldbach_count) -------- */
 
