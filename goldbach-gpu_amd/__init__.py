@@ -35,6 +35,7 @@ EXPORTS = (
     "gb_open", "gb_close", "gb_last_error", "gb_device_count", "gb_device_info",
     "gb_sieve_window", "gb_verify_segment", "gb_verify_range", "gb_verify_windows",
     "gb_min_partition_batch", "gb_autotune", "gb_count_partitions", "gb_verify_range_logged", "gb_log_read",
+    "gb_big_min_partition", "gb_big_from_decimal", "gb_big_to_decimal",
     "gb_serve_open", "gb_serve_submit", "gb_serve_poll", "gb_serve_stats_get", "gb_serve_close",
     "gb_serve_last_error",
 )
@@ -100,6 +101,14 @@ class GbLogInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class GbBigStats(ctypes.Structure):
+    _fields_ = [("primes_scanned", u64), ("candidates", u64), ("mr_tasks", u64), ("batches", u32), ("words", u32),
+                ("kernel_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # gb_result as a numpy record (48 bytes, same field order and widths)
 RESULT_DTYPE = np.dtype([("id", "<u8"), ("n", "<u8"), ("p", "<u8"), ("submit_ns", "<u8"), ("done_ns", "<u8"),
                          ("steps", "<u4"), ("reserved", "<u4")])
@@ -141,6 +150,10 @@ def library() -> ctypes.CDLL:
         "gb_count_partitions": (ctypes.c_int, [vp, u64, u64, vp]),
         "gb_verify_range_logged": (ctypes.c_int, [vp, u64, u64, u64, ctypes.POINTER(GbTuning), ctypes.c_char_p, u64,
                                                   u64, ctypes.POINTER(GbSegStats), ctypes.POINTER(GbLogInfo), vp]),
+        "gb_big_min_partition": (ctypes.c_int, [vp, vp, u32, u32, u64, ctypes.POINTER(u64),
+                                                ctypes.POINTER(GbBigStats)]),
+        "gb_big_from_decimal": (ctypes.c_int, [ctypes.c_char_p, vp, u32, ctypes.POINTER(u32)]),
+        "gb_big_to_decimal": (ctypes.c_int, [vp, u32, ctypes.c_char_p, ctypes.c_size_t]),
         "gb_log_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(GbLogHeader), ctypes.POINTER(GbLogRecord),
                                        u64, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
         "gb_autotune": (ctypes.c_int, [vp, u64, u64, u64, u32, u32, ctypes.POINTER(GbTuning),
@@ -158,6 +171,34 @@ def library() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def big_words(n: int) -> np.ndarray:
+    """n >= 0 as little-endian 32-bit words (at least one)."""
+    if n < 0:
+        raise ValueError("n must be >= 0")
+    nw = max(1, (n.bit_length() + 31) // 32)
+    return np.frombuffer(n.to_bytes(4 * nw, "little"), dtype=np.uint32).copy()
+
+
+def from_decimal(digits: str, cap: int = 1 << 16) -> np.ndarray:
+    """gb_big_from_decimal: decimal digits to little-endian 32-bit words."""
+    w = np.zeros(cap, dtype=np.uint32)
+    nw = u32(0)
+    rc = library().gb_big_from_decimal(digits.encode(), _ptr(w), cap, ctypes.byref(nw))
+    if rc != GB_OK:
+        raise GoldbachError(rc, f"gb_big_from_decimal failed for {digits[:20]!r}")
+    return w[:nw.value].copy()
+
+
+def to_decimal(words) -> str:
+    """gb_big_to_decimal: little-endian 32-bit words to decimal digits."""
+    w = np.ascontiguousarray(np.asarray(words, dtype=np.uint32))
+    buf = ctypes.create_string_buffer(10 * len(w) + 2)
+    rc = library().gb_big_to_decimal(_ptr(w), len(w), buf, len(buf))
+    if rc != GB_OK:
+        raise GoldbachError(rc, "gb_big_to_decimal failed")
+    return buf.value.decode()
 
 
 def read_log(path: str) -> tuple:
@@ -272,6 +313,16 @@ class Context:
                                                      os.fsencode(log_path), chunk_evens, max_chunks,
                                                      ctypes.byref(st), ctypes.byref(info), None))
         return st.as_dict(), info.as_dict()
+
+    def big_min_partition(self, n: int, rounds: int = 25, p_max: int = 0) -> tuple:
+        """(smallest prime p <= p_max with n - p a strong probable prime to the first `rounds`
+        prime bases, 0 if none; stats dict) for an even n >= 4 of any size up to 73,728 bits."""
+        w = big_words(n)
+        p = u64(0)
+        st = GbBigStats()
+        self._check(self._lib.gb_big_min_partition(self._h, _ptr(w), len(w), rounds, p_max, ctypes.byref(p),
+                                                   ctypes.byref(st)))
+        return p.value, st.as_dict()
 
     def autotune(self, start: int, limit: int, p_small: int, form: int = GB_FORM_BATCHED,
                  want_pmin: bool = False) -> tuple:

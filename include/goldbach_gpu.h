@@ -212,6 +212,35 @@ int gb_verify_range_logged(gb_ctx *ctx, uint64_t start, uint64_t limit, uint64_t
 int gb_log_read(const char *path, gb_log_header *header, gb_log_record *records, uint64_t cap, uint64_t *count,
                 uint32_t *torn);
 
+/* ---- arbitrary-size single n (big_check.cpp:35-144) -------------------- */
+
+typedef struct gb_big_stats {
+    uint64_t primes_scanned;   /* odd primes p up to the answer (all of them to p_max if none) */
+    uint64_t candidates;       /* of those, p with n - p free of prime factors below 2^20 */
+    uint64_t mr_tasks;         /* strong tests run on the device ((p, base) pairs) */
+    uint32_t batches;          /* test launches */
+    uint32_t words;            /* 32-bit words of n */
+    double   kernel_ms;
+} gb_big_stats;
+
+/* *p_out = the smallest prime p <= p_max (0: 10^7, big_check.cpp:58) such
+ * that n - p is a strong probable prime to each of the first `rounds` prime
+ * bases 2, 3, 5, ..., 97 (0: all 25); 0 when there is none.  n is even,
+ * >= 4, given as nwords little-endian 32-bit words (at most 2304: 73,728
+ * bits).  The reference asks GMP's mpz_probab_prime_p(q, 25)
+ * (big_check.cpp:107) for p in batches over threads and reports the first
+ * thread's p; here p is the smallest.  Below 2^64 the bases 2..37 decide
+ * primality exactly and the call answers through gb_min_partition_batch. */
+int gb_big_min_partition(gb_ctx *ctx, const uint32_t *n_words, uint32_t nwords, uint32_t rounds, uint64_t p_max,
+                         uint64_t *p_out, gb_big_stats *stats);
+
+/* Decimal digits <-> little-endian 32-bit words (no device needed).
+ * from_decimal: GB_EINVAL on an empty string or a non-digit, GB_ERANGE when
+ * more than cap words are needed.  to_decimal: GB_ERANGE when cap bytes
+ * cannot hold the digits and the terminating zero. */
+int gb_big_from_decimal(const char *digits, uint32_t *words, uint32_t cap, uint32_t *nwords);
+int gb_big_to_decimal(const uint32_t *words, uint32_t nwords, char *out, size_t cap);
+
 /* ---- partition counts (cpu_goldbach.cpp:29-36, goldbach_count) -------- */
 
 /* counts_out[i] = c(n) for n = start + 2i, i < count: the number of primes

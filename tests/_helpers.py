@@ -178,3 +178,46 @@ class Ref:
 
 def unpack(words, nbits):
     return np.unpackbits(np.ascontiguousarray(words).view(np.uint8), bitorder="little")[:nbits]
+
+
+def load_bigcheck():
+    """oracle/bigcheck.py: the big-n search restated in Python (test infrastructure)."""
+    name = "oracle_bigcheck"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "oracle", "bigcheck.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class Gmp:
+    """mpz_probab_prime_p of the image's GMP (the reference's big_check dependency,
+    big_check.cpp:107), through ctypes.  mpz_t is {int alloc; int size; limb *d}."""
+
+    class _Mpz(ctypes.Structure):
+        _fields_ = [("alloc", ctypes.c_int), ("size", ctypes.c_int), ("d", ctypes.c_void_p)]
+
+    def __init__(self):
+        lib = ctypes.CDLL("libgmp.so.10")
+        P = ctypes.POINTER(self._Mpz)
+        fn = lambda name: getattr(lib, name)       # names start with two underscores
+        self._init, self._clear = fn("__gmpz_init"), fn("__gmpz_clear")
+        self._set_str, self._pp = fn("__gmpz_set_str"), fn("__gmpz_probab_prime_p")
+        self._init.argtypes = [P]
+        self._clear.argtypes = [P]
+        self._set_str.argtypes = [P, ctypes.c_char_p, ctypes.c_int]
+        self._set_str.restype = ctypes.c_int
+        self._pp.argtypes = [P, ctypes.c_int]
+        self._pp.restype = ctypes.c_int
+        self.version = ctypes.c_char_p.in_dll(lib, "__gmp_version").value.decode()
+
+    def probab_prime(self, q: int, reps: int = 25) -> int:
+        z = self._Mpz()
+        self._init(ctypes.byref(z))
+        try:
+            assert self._set_str(ctypes.byref(z), format(q, "x").encode(), 16) == 0
+            return self._pp(ctypes.byref(z), reps)
+        finally:
+            self._clear(ctypes.byref(z))
