@@ -9,7 +9,7 @@
 
 namespace gbk {
 
-constexpr uint32_t BIG_BS = 256;       // threads per test
+constexpr uint32_t BIG_BS = 1024;      // threads per test, at most
 constexpr uint32_t BIG_LMAX = 2304;    // 32-bit words of n: 73,728 bits, 22,195 decimal digits
 
 struct BigTask {
@@ -17,8 +17,9 @@ struct BigTask {
     uint32_t base;   // strong-test base
 };
 
-// LDS bytes of one test workgroup for an n of L words.
+// LDS bytes of one test workgroup for an n of L words, and its threads.
 size_t big_lds_bytes(uint32_t L);
+uint32_t big_threads(uint32_t L);
 
 // pass[i] = 1 when n - tasks[i].p is a strong probable prime to base tasks[i].base
 // (n: L >= 2 words, little-endian, n - p >= 2^32 so q keeps at least two words).
