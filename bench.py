@@ -22,6 +22,9 @@ The JSON line carries:
   partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
                 candidate pairs/s (the reference loop's p = 2..n/2 per n), with
                 the oracle's c(n) on 512 n below 3e7 as its CPU baseline.
+  big_check     gb_big_min_partition for 10^1000 and 10^10000 (seconds per n,
+                against the reference's published 20-thread GMP times), with GMP's
+                own test over the same primes as its CPU baseline.
   serving       the continuous-batching server (gb_serve_*): p50/p99 latency
                 from scheduled arrival to result under open-loop Poisson
                 arrivals at 1e4, 1e5 and 1e6 queries/s, and queries/s for a
@@ -229,6 +232,48 @@ def cpu_counts_baseline() -> dict:
             "sample": f"c(n) for the 512 even n below 3e7 (sieve included), {threads} threads, {dt:.2f} s wall"}
 
 
+BIG_CASES = {1000: 26981, 10000: 47717}               # RESULTS.md:145-148 (the 10^1000 row run on one thread)
+BIG_REF_SECONDS = {1000: 0.363, 10000: 182.0}         # RESULTS.md:146,148: 20 threads, GMP
+
+
+def big_leg(ctx) -> dict:
+    """big_check on the device: the smallest p with 10^k - p a strong probable prime to 25 bases."""
+    if hasattr(sys, "set_int_max_str_digits"):
+        sys.set_int_max_str_digits(0)
+    ctx.big_min_partition(10**100)                       # warm (code, first launches)
+    out = {}
+    for k, want in BIG_CASES.items():
+        t0 = time.perf_counter()
+        p, st = ctx.big_min_partition(10**k)
+        dt = time.perf_counter() - t0
+        assert p == want, (k, p)
+        out[f"10^{k}"] = {"p": p, "seconds": dt, "kernel_ms": st["kernel_ms"], "strong_tests": st["mr_tasks"],
+                          "words": st["words"], "reference_seconds_20_threads": BIG_REF_SECONDS[k],
+                          "vs_reference": BIG_REF_SECONDS[k] / dt}
+    return out
+
+
+def cpu_big_baseline() -> dict:
+    """GMP's mpz_probab_prime_p(10^1000 - p, 25), the reference's test (big_check.cpp:107), over the
+    primes p <= 26,981 in order, on a thread pool (ctypes releases the GIL during the call)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from _helpers import Gmp, load_bigcheck
+
+    gmp = Gmp()
+    primes = [p for p in load_bigcheck().primes_upto(BIG_CASES[1000])]
+    n = 10**1000
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        verdicts = list(ex.map(lambda p: gmp.probab_prime(n - p, 25), primes))
+    dt = time.perf_counter() - t0
+    assert verdicts[-1] > 0 and not any(verdicts[:-1])
+    return {"value": dt, "unit": "s for 10^1000", "cores": threads, "kind": "port",
+            "sample": f"GMP {gmp.version} mpz_probab_prime_p(10^1000 - p, 25) for the {len(primes)} primes "
+                      f"p <= 26,981, {threads} threads"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +284,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-windows", action="store_true", help="skip the high-window (Miller-Rabin) leg")
     ap.add_argument("--no-counts", action="store_true", help="skip the partition-count leg")
+    ap.add_argument("--no-big", action="store_true", help="skip the big_check (10^1000, 10^10000) leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -292,10 +338,13 @@ def main() -> int:
     serve = None if args.no_serve else serving(ctx)
     high = None if args.no_windows else windows_leg(ctx)
     counts = None if args.no_counts else counts_leg(ctx)
+    big = None if args.no_big else big_leg(ctx)
     ctx.close()
     cpu = None if args.no_cpu else cpu_baseline()
     if counts is not None and not args.no_cpu:
         counts["cpu_baseline"] = cpu_counts_baseline()
+    if big is not None and not args.no_cpu:
+        big["cpu_baseline"] = cpu_big_baseline()
 
     line = {
         "metric": METRIC,
@@ -324,6 +373,7 @@ def main() -> int:
         "serving": serve,
         "high_windows": high,
         "partition_counts": counts,
+        "big_check": big,
     }
     print(json.dumps(line))
     return 0
