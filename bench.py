@@ -16,7 +16,7 @@ The JSON line carries:
                 committed rocprofv3 counter summary when present.
   cpu_baseline  the CPU restatement (oracle/, OpenMP over segments, the
                 reference algorithm: per-n ascending prime scan) timed on this
-                host over 32 sampled 2^24-even segments spread over [4, 1e12].
+                host over 64 sampled 2^24-even segments spread over [4, 1e12].
   high_windows  64 windows above 1e15 (q by pre-sieve + Miller-Rabin) in one
                 persistent launch, evens/s.
   partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
@@ -77,7 +77,7 @@ def cpu_baseline() -> dict:
     orc = Oracle()
     threads = min(16, os.cpu_count() or 1)
     seg = 1 << 24
-    starts = [4 + 2 * ((k * (LIMIT // 2 - seg)) // 31) for k in range(32)]
+    starts = [4 + 2 * ((k * (LIMIT // 2 - seg)) // 63) for k in range(64)]
     evens = 0
     t0 = time.perf_counter()
     from concurrent.futures import ThreadPoolExecutor
@@ -92,7 +92,7 @@ def cpu_baseline() -> dict:
             assert f == 0
     dt = time.perf_counter() - t0
     return {"value": evens / dt, "unit": "evens/s", "cores": threads, "kind": "port",
-            "sample": f"32 segments of 2^24 evens spread over [4, 1e12], {threads} threads, {dt:.1f} s wall"}
+            "sample": f"64 segments of 2^24 evens spread over [4, 1e12], {threads} threads, {dt:.1f} s wall"}
 
 
 def serving_ns(count: int, seed: int = 1) -> np.ndarray:

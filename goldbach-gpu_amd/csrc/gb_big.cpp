@@ -13,6 +13,7 @@
 // Every smaller p failed a test or has a small factor in n - p, so the p
 // returned is the smallest whose n - p passes all `rounds` bases.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -168,7 +169,8 @@ int gb_big_min_partition(gb_ctx* c, const uint32_t* n_words, uint32_t nwords, ui
     HCK(c, hipMemcpyAsync(d_ell.p, ell.data(), ell.size() * 4, hipMemcpyHostToDevice, s));
     HCK(c, hipMemsetAsync(d_bad.p, 0, bad_words * 4, s));
     HCK(c, hipEventRecord(c->ev0, s));
-    HCK(c, gbk::launch_big_filter(s, d_n.p, L, d_ell.p, (uint32_t)ell.size(), d_r.p, p_max, d_bad.p));
+    const uint32_t nsmall = (uint32_t)(std::lower_bound(ell.begin(), ell.end(), 1u << 13) - ell.begin());
+    HCK(c, gbk::launch_big_filter(s, d_n.p, L, d_ell.p, (uint32_t)ell.size(), nsmall, d_r.p, p_max, d_bad.p));
     HCK(c, hipEventRecord(c->ev1, s));
     std::vector<uint32_t> bad(bad_words);
     HCK(c, hipMemcpyAsync(bad.data(), d_bad.p, bad_words * 4, hipMemcpyDeviceToHost, s));
@@ -203,10 +205,16 @@ int gb_big_min_partition(gb_ctx* c, const uint32_t* n_words, uint32_t nwords, ui
         ++stats.batches;
         return GB_OK;
     };
+    // A candidate (n - p free of factors below 2^20) is prime with probability about
+    // 1 / (ln n * prod_(ell < 2^20) (1 - 1/ell)) = 1 / (0.0405 ln n): the first batch takes three
+    // times that many (a 5 % chance of needing a second), at least one per CU, at most a full batch.
+    const double ln_n = (32.0 * (L - 1) + std::log2((double)n_words[L - 1] + 1.0)) * 0.6931471805599453;
+    const uint32_t first =
+        (uint32_t)std::min<double>(batch, std::max<double>(std::max(1, c->cus), 3.0 * 0.0405 * ln_n));
     uint64_t found = 0;
     size_t next = 0;
     while (!found && next < cand.size()) {
-        const uint32_t cnt = (uint32_t)std::min<size_t>(batch, cand.size() - next);
+        const uint32_t cnt = (uint32_t)std::min<size_t>(next == 0 ? first : batch, cand.size() - next);
         for (uint32_t i = 0; i < cnt; ++i) tasks[i] = gbk::BigTask{cand[next + i], 2};
         int rc = run(cnt);
         if (rc) return rc;

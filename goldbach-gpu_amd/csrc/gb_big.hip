@@ -511,16 +511,18 @@ __global__ void big_residue_kernel(const uint32_t* __restrict__ n, uint32_t L, c
     r[i] = (uint32_t)v;
 }
 
-// bad bit (p - 1) / 2 for every odd p <= p_max with p = n mod ell, i.e. ell | n - p
-__global__ void big_filter_kernel(const uint32_t* __restrict__ ell, const uint32_t* __restrict__ r, uint32_t nell,
-                                  uint64_t p_max, uint32_t* __restrict__ bad) {
-    const uint32_t i = blockIdx.x;
-    if (i >= nell) return;
-    const uint64_t l = ell[i];
-    uint64_t p = r[i];
-    if (!(p & 1)) p += l;                            // odd p only (l odd)
-    for (p += 2 * l * threadIdx.x; p <= p_max; p += 2 * l * blockDim.x)
-        atomicOr(&bad[p >> 6], 1u << ((p >> 1) & 31));
+// bad bit (p - 1) / 2 for every odd p <= p_max with p = n mod ell, i.e. ell | n - p.
+// Block (x, y): prime ell[i0 + x], the p of [y span, (y + 1) span).
+__global__ void big_filter_kernel(const uint32_t* __restrict__ ell, const uint32_t* __restrict__ r, uint32_t i0,
+                                  uint64_t p_max, uint64_t span, uint32_t* __restrict__ bad) {
+    const uint32_t i = i0 + blockIdx.x;
+    const uint64_t l = ell[i], l2 = 2 * l;
+    const uint64_t ro = (r[i] & 1) ? r[i] : r[i] + l;   // the odd p = n mod ell: p = ro mod 2 ell
+    const uint64_t lo = blockIdx.y * span;
+    if (lo > p_max) return;
+    const uint64_t hi = lo + span - 1 < p_max ? lo + span - 1 : p_max;
+    uint64_t p = lo + (ro + l2 - lo % l2) % l2;
+    for (p += l2 * threadIdx.x; p <= hi; p += l2 * blockDim.x) atomicOr(&bad[p >> 6], 1u << ((p >> 1) & 31));
 }
 
 size_t big_lds_bytes(uint32_t L) { return (size_t)layout(L).total * 4; }
@@ -541,12 +543,21 @@ hipError_t launch_big_mr(hipStream_t s, const uint32_t* n, uint32_t L, const Big
 }
 
 hipError_t launch_big_filter(hipStream_t s, const uint32_t* n, uint32_t L, const uint32_t* ell, uint32_t nell,
-                             uint32_t* r, uint64_t p_max, uint32_t* bad) {
+                             uint32_t nsmall, uint32_t* r, uint64_t p_max, uint32_t* bad) {
     if (nell == 0) return hipSuccess;
     hipLaunchKernelGGL(big_residue_kernel, dim3((nell + 255) / 256), dim3(256), 0, s, n, L, ell, nell, r);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(big_filter_kernel, dim3(nell), dim3(64), 0, s, ell, r, nell, p_max, bad);
+    // the first nsmall primes hit many p: their ranges are cut in 64 pieces
+    const uint64_t span = (p_max + 64) / 64;
+    if (nsmall) {
+        hipLaunchKernelGGL(big_filter_kernel, dim3(nsmall, 64), dim3(64), 0, s, ell, r, 0u, p_max, span, bad);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nell > nsmall)
+        hipLaunchKernelGGL(big_filter_kernel, dim3(nell - nsmall, 1), dim3(64), 0, s, ell, r, nsmall, p_max,
+                           p_max + 1, bad);
     return hipGetLastError();
 }
 

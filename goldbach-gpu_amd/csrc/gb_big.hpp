@@ -27,8 +27,9 @@ hipError_t launch_big_mr(hipStream_t s, const uint32_t* n, uint32_t L, const Big
                          uint32_t* pass);
 
 // bad: bit (p - 1)/2 set for every odd p <= p_max with some ell[i] dividing n - p
-// (r: nell words of scratch, bad: zeroed, p_max / 64 + 1 words).
+// (ell ascending, the first nsmall of them below 2^13; r: nell words of scratch;
+// bad: zeroed, p_max / 64 + 1 words).
 hipError_t launch_big_filter(hipStream_t s, const uint32_t* n, uint32_t L, const uint32_t* ell, uint32_t nell,
-                             uint32_t* r, uint64_t p_max, uint32_t* bad);
+                             uint32_t nsmall, uint32_t* r, uint64_t p_max, uint32_t* bad);
 
 }  // namespace gbk
