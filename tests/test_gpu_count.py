@@ -8,7 +8,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-C_POW10 = (2, 6, 28, 127, 810, 5402, 38807, 291400, 2274205)   # OEIS A065577, k = 1..9
+C_POW10 = (2, 6, 28, 127, 810, 5402, 38807, 291400, 2274205, 18200488)   # OEIS A065577, k = 1..10
 
 
 @pytest.mark.parametrize("start,count", [(4, 1), (4, 2), (6, 1), (4, 5000), (8, 4097), (1_000_000, 3000),
