@@ -13,8 +13,11 @@
 //   goldbach-run-log 1 start=S limit=L p_small=P phase2_limit=Q chunk_evens=C pmin=F crc=XXXXXXXX
 //   chunk=I first=A last=B evens=E phase2=X failures=Y first_fail=Z sum_pmin=U max_pmin=V argmax=W crc=XXXXXXXX
 #include <fcntl.h>
+#include <sys/file.h>
 #include <unistd.h>
 
+#include <array>
+#include <cerrno>
 #include <cinttypes>
 #include <cstdio>
 #include <cstring>
@@ -30,16 +33,16 @@ namespace {
 constexpr uint64_t DEFAULT_CHUNK_EVENS = 1ull << 36;
 
 uint32_t crc32_ieee(const char* s, size_t n) {
-    static uint32_t table[256];
-    static bool ready = false;
-    if (!ready) {   // reflected polynomial 0xEDB88320, one table entry per byte value
+    // reflected polynomial 0xEDB88320, one table entry per byte value (built once, thread-safe)
+    static const std::array<uint32_t, 256> table = [] {
+        std::array<uint32_t, 256> t{};
         for (uint32_t i = 0; i < 256; ++i) {
             uint32_t c = i;
             for (int k = 0; k < 8; ++k) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-            table[i] = c;
+            t[i] = c;
         }
-        ready = true;
-    }
+        return t;
+    }();
     uint32_t c = 0xFFFFFFFFu;
     for (size_t i = 0; i < n; ++i) c = table[(c ^ (uint8_t)s[i]) & 0xFF] ^ (c >> 8);
     return c ^ 0xFFFFFFFFu;
@@ -242,24 +245,34 @@ int gb_verify_range_logged(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p
     want.chunk_evens = chunk_evens ? chunk_evens : DEFAULT_CHUNK_EVENS;
     want.want_pmin = (flags & GB_WANT_PMIN) ? 1 : 0;
 
+    // one writer at a time: the lock is held from the read to the last record
+    const int fd = ::open(path, O_WRONLY | O_CREAT, 0644);
+    if (fd < 0) return fail(c, GB_EIO, std::string("cannot open run log ") + path + ": " + std::strerror(errno));
+    if (::flock(fd, LOCK_EX | LOCK_NB) != 0) {
+        ::close(fd);
+        return fail(c, GB_EAGAIN, std::string("run log ") + path + " is in use by another run");
+    }
+    auto io_fail = [&](const char* what) {
+        const std::string m = std::string(what) + " " + path + ": " + std::strerror(errno);
+        ::close(fd);
+        return fail(c, GB_EIO, m);
+    };
+
     bool exists = false;
     gb_log_header have{};
     std::vector<gb_log_record> recs;
     uint64_t valid = 0, total = 0;
     std::string why;
     rc = read_log(path, &exists, &have, &recs, &valid, &total, &why);
-    if (rc) return fail(c, rc, std::string(path) + ": " + why);
-    if (exists && header_body(have) != header_body(want))
+    if (rc) {
+        ::close(fd);
+        return fail(c, rc, std::string(path) + ": " + why);
+    }
+    if (exists && header_body(have) != header_body(want)) {
+        ::close(fd);
         return fail(c, GB_EINVAL, std::string(path) + " was written for another run (" + header_body(have) +
                                       "); this run is (" + header_body(want) + ")");
-
-    const int fd = ::open(path, O_WRONLY | O_CREAT, 0644);
-    if (fd < 0) return fail(c, GB_EIO, std::string("cannot open run log ") + path + ": " + std::strerror(errno));
-    auto io_fail = [&](const char* what) {
-        const std::string m = std::string(what) + " " + path + ": " + std::strerror(errno);
-        ::close(fd);
-        return fail(c, GB_EIO, m);
-    };
+    }
     // keep the intact prefix (drops a torn or corrupt tail), then append
     if (::ftruncate(fd, (off_t)valid) != 0) return io_fail("cannot truncate run log");
     if (::lseek(fd, 0, SEEK_END) < 0) return io_fail("cannot seek run log");

@@ -182,7 +182,8 @@ int gb_min_partition_batch(gb_ctx *ctx, const uint64_t *n, uint32_t count, uint6
  * is refused with GB_EINVAL.  A chunk with a failure ends the run, as the
  * reference stops at its first failure (goldbach.cu:476-480).
  * max_chunks > 0 stops after that many new chunks (a time-sliced job);
- * *out covers every chunk in the log, its kernel_ms only this call's. */
+ * *out covers every chunk in the log, its kernel_ms only this call's.  One
+ * writer at a time: a log locked by another call returns GB_EAGAIN. */
 typedef struct gb_log_header {
     uint64_t start, limit, p_small, phase2_limit, chunk_evens;
     uint32_t want_pmin, reserved;

@@ -5,6 +5,7 @@ and reaches the same totals; a log written for another run is refused; a
 chunk with a failure ends the run as the reference stops at its first failure
 (goldbach.cu:476-480); the CLI's --log resumes instead of --start
 (goldbach.cu:581,604-609,689)."""
+import fcntl
 import os
 import subprocess
 
@@ -120,3 +121,14 @@ def test_cli_log_flag(tmp_path):
     r2 = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r2.returncode == 0 and "(3 of 7 chunks resumed, 900000000 evens)" in r2.stdout, r2.stdout
     assert open(path).read() == "".join(lines)
+
+
+def test_one_writer_at_a_time(ctx, gb, tmp_path):
+    path = str(tmp_path / "l.log")
+    with open(path, "w") as f:
+        fcntl.flock(f, fcntl.LOCK_EX)                    # another run holds the log
+        with pytest.raises(gb.GoldbachError) as e:
+            ctx.verify_range_logged(4, 10**6, 10**5, path, chunk_evens=10**5)
+        assert e.value.status == gb.GB_EAGAIN and "in use" in e.value.message
+    st, info = ctx.verify_range_logged(4, 10**6, 10**5, path, chunk_evens=10**5)
+    assert info["complete"] == 1 and st["evens"] == (10**6 - 4) // 2 + 1
