@@ -193,11 +193,11 @@ __device__ __forceinline__ Mont mont_setup(uint64_t n) {
     return M;
 }
 
-// Strong probable-prime test to base a (a < n), n odd > 2, n - 1 = d * 2^r.
+// Strong probable-prime test to base a (0 < a < n, a < 2^32), n odd > 2, n - 1 = d * 2^r.
 __device__ __forceinline__ bool sprp(uint64_t a, uint64_t d, uint32_t r, const Mont& M) {
-    // a in Montgomery form: a * R mod n by double-and-add (a <= 37)
+    // a in Montgomery form: a * R mod n by double-and-add over a's bits
     uint64_t am = 0;
-    for (int b = 5; b >= 0; --b) {
+    for (int b = 31 - __clz((uint32_t)a); b >= 0; --b) {
         am = add_mod(am, am, M.n);
         if ((a >> b) & 1) am = add_mod(am, M.one, M.n);
     }
@@ -235,13 +235,23 @@ __device__ __forceinline__ bool sprp2(uint64_t d, uint32_t r, const Mont& M) {
     return false;
 }
 
-// The Miller-Rabin bases 2, 3, 5, ..., 37 in order.  The first 12 decide
-// every n < 2^64 (Sorenson & Webster 2015).  With fewer leading bases the
-// test is already exact below psi_k, the smallest odd composite passing the
-// first k of them (OEIS A014233), so mr_bases(n) of them give the 12-base answer.
-__device__ __forceinline__ uint32_t mr_base(uint32_t k) {
+// Miller-Rabin bases.  Below psi_7 = 341550071728321: the prime bases 2, 3, 5, ... in
+// order, the first k of which are exact below psi_k, the smallest odd composite
+// passing all k (OEIS A014233).  From psi_7 up: Sinclair's seven bases 2, 325, 9375,
+// 28178, 450775, 9780504, 1795265022, exact for every n < 2^64 (J. Sinclair, 2011; every
+// base is below n there, so none reduces to 0).  Both give the answer of the twelve prime
+// bases 2..37 (Sorenson & Webster 2015).
+constexpr uint64_t MR_PSI7 = 341550071728321ull;
+
+__device__ __forceinline__ uint32_t mr_base(uint32_t k, bool wide) {
+    if (wide) {
+        constexpr uint32_t S[7] = {2u, 325u, 9375u, 28178u, 450775u, 9780504u, 1795265022u};
+        return S[k];
+    }
     return k < 6 ? (0x0D0B07050302ull >> (8 * k)) & 0xFF : (0x251F1D171311ull >> (8 * (k - 6))) & 0xFF;
 }
+
+__host__ __device__ inline bool mr_wide(uint64_t n) { return n >= MR_PSI7; }
 
 __host__ __device__ inline uint32_t mr_bases(uint64_t n) {
     if (n < 2047ull) return 1;
@@ -250,18 +260,16 @@ __host__ __device__ inline uint32_t mr_bases(uint64_t n) {
     if (n < 3215031751ull) return 4;
     if (n < 2152302898747ull) return 5;
     if (n < 3474749660383ull) return 6;
-    if (n < 341550071728321ull) return 7;
-    if (n < 3825123056546413051ull) return 9;
-    return 12;
+    return 7;
 }
 
-// Strong test of odd n > 37 to the k-th base (k = 0: base 2).
-__device__ inline bool mr_test(uint64_t n, uint32_t k) {
+// Strong test of odd n > 37 to base k of the set `wide` selects (k = 0: base 2).
+__device__ inline bool mr_test(uint64_t n, uint32_t k, bool wide) {
     const Mont M = mont_setup(n);
     uint64_t d = n - 1;
     const uint32_t r = __builtin_ctzll(d);
     d >>= r;
-    return k == 0 ? sprp2(d, r, M) : sprp(mr_base(k), d, r, M);
+    return k == 0 ? sprp2(d, r, M) : sprp(mr_base(k, wide), d, r, M);
 }
 
 // Deterministic for every 64-bit n.  Small n (< 41) are answered directly;
@@ -276,9 +284,10 @@ __device__ inline bool is_prime_u64(uint64_t n) {
     d >>= r;
     if (!sprp2(d, r, M)) return false;
     const uint32_t nb = mr_bases(n);
+    const bool wide = mr_wide(n);
 #pragma unroll 1
     for (uint32_t i = 1; i < nb; ++i)
-        if (!sprp(mr_base(i), d, r, M)) return false;
+        if (!sprp(mr_base(i, wide), d, r, M)) return false;
     return true;
 }
 

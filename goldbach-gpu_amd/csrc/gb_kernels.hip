@@ -118,13 +118,16 @@ __device__ uint32_t block_scan(uint32_t c, uint32_t tid, uint32_t* __restrict__ 
 //   pass 1: every (surviving q, base k) pair for k = 1 .. mr_bases(q_top)-1 is
 //           one lane's test;
 // a failed test clears the bit.  Extra bases never change a correct answer,
-// so the tile's largest q sets the base count for all of its q.
+// so the tile's largest q sets the base set and count for all of its q (the
+// seven wide bases are exact for every q < 2^64).
 template <int BS>
 __device__ void mr_tile(uint32_t* __restrict__ win32, uint32_t nbits, uint64_t qb, uint32_t tid,
                         uint32_t* __restrict__ queue, uint32_t qcap, uint32_t* __restrict__ wsum) {
     const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
-    const uint32_t nb = mr_bases(qb + 2ull * (nbits - 1));
+    const uint64_t q_top = qb + 2ull * (nbits - 1);
+    const uint32_t nb = mr_bases(q_top);
+    const bool wide = mr_wide(q_top);
     for (uint32_t pass = 0; pass < 2; ++pass) {
         const uint32_t per = pass == 0 ? 1u : nb - 1;   // tests per queued q
         if (per == 0) break;
@@ -143,7 +146,7 @@ __device__ void mr_tile(uint32_t* __restrict__ win32, uint32_t nbits, uint64_t q
                 for (uint32_t u = tid; u < n * per; u += BS) {
                     const uint32_t j = u / per;
                     const uint32_t i = queue[j];
-                    if (!mr_test(qb + 2ull * i, pass == 0 ? 0u : u - j * per + 1)) lds_clear_bit(win32, i);
+                    if (!mr_test(qb + 2ull * i, pass == 0 ? 0u : u - j * per + 1, wide)) lds_clear_bit(win32, i);
                 }
                 __syncthreads();
             }
@@ -1420,7 +1423,7 @@ __device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint6
         if (q <= T.p2_limit) {
             pass = is_prime_q(q, T);
         } else if (!small_factor(q)) {
-            pass = mr_test(q, 0);
+            pass = mr_test(q, 0, false);
         }
     }
     uint64_t ball = __ballot(pass);
@@ -1428,7 +1431,7 @@ __device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint6
         const uint32_t L = __ffsll((unsigned long long)ball) - 1;
         const uint64_t qq = __shfl(q, L);
         bool ok = true;
-        if (qq > T.p2_limit && lane >= 1 && lane < mr_bases(qq)) ok = mr_test(qq, lane);
+        if (qq > T.p2_limit && lane >= 1 && lane < mr_bases(qq)) ok = mr_test(qq, lane, mr_wide(qq));
         if (__all(ok)) {
             res = __shfl(p, L);
             return 1;

@@ -308,3 +308,38 @@ def test_count_partitions_positive_and_parity_sum(orc):
             if (p + q) % 2 == 0:
                 pairs += 1
     assert int(got.sum()) == pairs
+
+
+# Sinclair's seven bases, the device's Miller-Rabin set from psi_7 = 341550071728321 up
+SINCLAIR = (2, 325, 9375, 28178, 450775, 9780504, 1795265022)
+
+
+def test_wide_bases_agree_with_the_twelve_prime_bases(orc):
+    """The device's wide base set against the oracle's twelve prime bases 2..37 on the
+    strong pseudoprimes of the psi table, semiprimes of two primes near 2^32 (the hard
+    composites for a strong test), the largest 64-bit prime, and random odd q up to 2^64."""
+    def strong(n, a):
+        d, s = n - 1, 0
+        while d % 2 == 0:
+            d //= 2
+            s += 1
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            return True
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                return True
+        return False
+
+    rng = np.random.default_rng(17)
+    qs = [psi for _, psi in PSI if psi >= 341550071728321]
+    qs += [int(x) | 1 for x in rng.integers(341550071728321, 2**63, size=4000, dtype=np.uint64)]
+    qs += [int(x) * 2 + 1 for x in rng.integers(2**62, 2**63 - 1, size=4000, dtype=np.uint64)]
+    near = [p for p in range(2**32 - 20000, 2**32) if trial_division(p)]
+    qs += [near[i] * near[-1 - i] for i in range(200)]
+    qs += [2**61 - 1, 2**64 - 59, (2**61 - 1) * 3 * 5 + 2]       # primes at the top, a near miss
+    for q in qs:
+        if q < 341550071728321 or q >= 2**64 or q % 2 == 0:
+            continue
+        assert all(strong(q, a) for a in SINCLAIR) == orc.is_prime(q), q
