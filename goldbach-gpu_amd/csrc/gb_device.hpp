@@ -272,6 +272,108 @@ __device__ inline bool mr_test(uint64_t n, uint32_t k, bool wide) {
     return k == 0 ? sprp2(d, r, M) : sprp(mr_base(k, wide), d, r, M);
 }
 
+// ---------------------------------------------------------------------------
+// Strong Lucas test (Baillie & Wagstaff 1980, Selfridge's parameters: the first D
+// of 5, -7, 9, -11, ... with Jacobi (D/n) = -1, P = 1, Q = (1 - D)/4).  With the
+// base-2 strong test it is the Baillie-PSW test, which no composite below 2^64
+// passes (Gilchrist 2013, over Feitsma's list of base-2 strong pseudoprimes).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t n) {
+    return a >= b ? a - b : a + (n - b);
+}
+
+__device__ __forceinline__ uint64_t half_mod(uint64_t x, uint64_t n) {   // x / 2 mod n, x < n, n odd
+    return (x & 1) ? (x >> 1) + (n >> 1) + 1 : x >> 1;
+}
+
+// v (|v| < 2^31, |v| < n) in Montgomery form
+__device__ inline uint64_t mont_small(int64_t v, const Mont& M) {
+    const uint32_t a = (uint32_t)(v < 0 ? -v : v);
+    uint64_t am = 0;
+    for (int b = 31 - __clz(a); b >= 0; --b) {
+        am = add_mod(am, am, M.n);
+        if ((a >> b) & 1) am = add_mod(am, M.one, M.n);
+    }
+    return (v < 0 && am) ? M.n - am : am;
+}
+
+// Jacobi symbol (a/m), m odd > 0
+__device__ inline int jacobi32(uint32_t a, uint32_t m) {
+    int t = 1;
+    a %= m;
+    while (a) {
+        while (!(a & 1)) {
+            a >>= 1;
+            if ((m & 7) == 3 || (m & 7) == 5) t = -t;
+        }
+        const uint32_t x = a;
+        a = m;
+        m = x;
+        if ((a & 3) == 3 && (m & 3) == 3) t = -t;
+        a %= m;
+    }
+    return m == 1 ? t : 0;
+}
+
+// Strong Lucas probable-prime test of odd n > 2^32 (a perfect square, for which no D
+// exists, is composite).  Returns 1 (passes), 0 (composite) or -1 when no D below 2^16
+// turned up (the first D is small for every non-square; the caller then decides by
+// Miller-Rabin).
+__device__ inline int strong_lucas(uint64_t n, const Mont& M) {
+    const uint64_t r = isqrt64(n);
+    if (r * r == n) return 0;
+    int64_t D = 5;
+    for (;;) {
+        const uint32_t ad = (uint32_t)(D < 0 ? -D : D);
+        if (ad >= 65536) return -1;
+        // (D/n) = (n mod |D| / |D|) by reciprocity, times (-1/n) for negative D
+        const uint32_t c32 = (0xFFFFFFFFu % ad + 1) % ad;                       // 2^32 mod |D|
+        const uint32_t nm = (((uint32_t)(n >> 32) % ad) * c32 + (uint32_t)n % ad) % ad;
+        int j = jacobi32(nm, ad);
+        if ((((ad - 1) >> 1) & 1) && (((n - 1) >> 1) & 1)) j = -j;
+        if (D < 0 && (n & 3) == 3) j = -j;
+        if (j == -1) break;
+        if (j == 0) return 0;                                                   // |D| < n shares a factor
+        D = D > 0 ? -D - 2 : -D + 2;
+    }
+    const uint64_t Dm = mont_small(D, M), Qm = mont_small((1 - D) / 4, M);
+    uint64_t d = n + 1;
+    const uint32_t s = __builtin_ctzll(d);
+    d >>= s;
+    uint64_t U = M.one, V = M.one, Qk = Qm;                                      // k = 1
+    for (int b = 62 - __clzll((long long)d); b >= 0; --b) {
+        U = mont_mul(U, V, M);                                                   // k -> 2k
+        V = sub_mod(mont_mul(V, V, M), add_mod(Qk, Qk, M.n), M.n);
+        Qk = mont_mul(Qk, Qk, M);
+        if ((d >> b) & 1) {                                                      // 2k -> 2k + 1
+            const uint64_t u1 = half_mod(add_mod(U, V, M.n), M.n);
+            V = half_mod(add_mod(mont_mul(Dm, U, M), V, M.n), M.n);
+            U = u1;
+            Qk = mont_mul(Qk, Qm, M);
+        }
+    }
+    if (U == 0 || V == 0) return 1;
+    for (uint32_t i = 1; i < s; ++i) {
+        V = sub_mod(mont_mul(V, V, M), add_mod(Qk, Qk, M.n), M.n);
+        if (V == 0) return 1;
+        Qk = mont_mul(Qk, Qk, M);
+    }
+    return 0;
+}
+
+// The Lucas half of Baillie-PSW for odd n > 2^32 that passed the base-2 strong test.
+__device__ inline bool bpsw_lucas(uint64_t n) {
+    const Mont M = mont_setup(n);
+    const int r = strong_lucas(n, M);
+    if (r >= 0) return r == 1;
+    uint64_t d = n - 1;
+    const uint32_t s = __builtin_ctzll(d);
+    d >>= s;
+    for (uint32_t k = 1; k < 7; ++k)
+        if (!sprp(mr_base(k, true), d, s, M)) return false;
+    return true;
+}
+
 // Deterministic for every 64-bit n.  Small n (< 41) are answered directly;
 // n even is composite.
 __device__ inline bool is_prime_u64(uint64_t n) {

@@ -110,27 +110,20 @@ __device__ uint32_t block_scan(uint32_t c, uint32_t tid, uint32_t* __restrict__ 
     return before + x - c;
 }
 
-// Miller-Rabin decision of the pre-sieve survivors of a tile (bit i <-> odd
-// q = qb + 2i, every q > 2^44).  Survivors are packed into an LDS queue so all
-// lanes run equal-length exponentiations instead of each lane walking its own
-// word's survivors:
+// Primality of the pre-sieve survivors of a tile (bit i <-> odd q = qb + 2i, every
+// q > 2^44) by Baillie-PSW.  Survivors are packed into an LDS queue so all lanes
+// run tests of equal length instead of each lane walking its own word's survivors:
 //   pass 0: every survivor gets the base-2 strong test,
-//   pass 1: every (surviving q, base k) pair for k = 1 .. mr_bases(q_top)-1 is
-//           one lane's test;
-// a failed test clears the bit.  Extra bases never change a correct answer,
-// so the tile's largest q sets the base set and count for all of its q (the
-// seven wide bases are exact for every q < 2^64).
+//   pass 1: every survivor left gets the strong Lucas test;
+// a failed test clears the bit.  No composite below 2^64 passes both, so the answer is
+// the one of the Miller-Rabin bases 2..37; nearly every base-2 survivor is prime, and
+// one Lucas test costs about as much as three more strong tests.
 template <int BS>
 __device__ void mr_tile(uint32_t* __restrict__ win32, uint32_t nbits, uint64_t qb, uint32_t tid,
                         uint32_t* __restrict__ queue, uint32_t qcap, uint32_t* __restrict__ wsum) {
     const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
-    const uint64_t q_top = qb + 2ull * (nbits - 1);
-    const uint32_t nb = mr_bases(q_top);
-    const bool wide = mr_wide(q_top);
     for (uint32_t pass = 0; pass < 2; ++pass) {
-        const uint32_t per = pass == 0 ? 1u : nb - 1;   // tests per queued q
-        if (per == 0) break;
         for (uint32_t wb = 0; wb < nw64; wb += BS) {
             const uint32_t k = wb + tid;
             uint64_t w = k < nw64 ? win64[k] : 0;
@@ -143,10 +136,10 @@ __device__ void mr_tile(uint32_t* __restrict__ win32, uint32_t nbits, uint64_t q
                     if (pos - base < qcap) queue[pos - base] = 64 * k + __builtin_ctzll(x);
                 __syncthreads();
                 const uint32_t n = total - base < qcap ? total - base : qcap;
-                for (uint32_t u = tid; u < n * per; u += BS) {
-                    const uint32_t j = u / per;
-                    const uint32_t i = queue[j];
-                    if (!mr_test(qb + 2ull * i, pass == 0 ? 0u : u - j * per + 1, wide)) lds_clear_bit(win32, i);
+                for (uint32_t u = tid; u < n; u += BS) {
+                    const uint32_t i = queue[u];
+                    const uint64_t q = qb + 2ull * i;
+                    if (!(pass == 0 ? mr_test(q, 0, false) : bpsw_lucas(q))) lds_clear_bit(win32, i);
                 }
                 __syncthreads();
             }
