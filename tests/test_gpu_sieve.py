@@ -21,14 +21,21 @@ WINDOWS = [
     (3825123056546413051 - 40_000, 3825123056546413051 + 40_000),    # ... to bases 2..31
 ]
 
-# strong pseudoprimes whose factors all exceed the 2^18 pre-sieve reach Miller-Rabin
-SPSP = [341550071728321]
+# Base-2 strong pseudoprimes whose factors all exceed the 2^18 pre-sieve reach the
+# tiles' second test (the strong Lucas test of Baillie-PSW): 10670053 * 32010157,
+# 4195117 * 8390233, 262957 * 525913 * 788869 (a (6k+1)(12k+1)(18k+1) Carmichael number).
+SPSP = [341550071728321, 35198009092261, 109094669922527929]
+# squares of primes above the pre-sieve: no Lucas parameter D exists for them
+SQUARES = [4194319**2, 2147483647**2]
 
 
-@pytest.mark.parametrize("q", SPSP)
-def test_strong_pseudoprime_is_cleared(ctx, q):
+@pytest.mark.parametrize("q", SPSP + SQUARES)
+def test_strong_pseudoprime_is_cleared(ctx, orc, q):
     bits = ctx.sieve_window(q - 2 * 64, q + 2 * 64)
     assert unpack(bits, 129)[64] == 0
+    assert orc.strong_first(q, 1) == (q in SPSP) and not orc.is_prime(q)
+    # and the window around it equals the oracle's
+    assert np.array_equal(ctx.sieve_window(q - 20_000, q + 20_000), orc.window_bits(q - 20_000, q + 20_000))
 
 
 @pytest.mark.parametrize("q_low,q_high", WINDOWS)
