@@ -17,7 +17,7 @@ The JSON line carries:
   cpu_baseline  the CPU restatement (oracle/, OpenMP over segments, the
                 reference algorithm: per-n ascending prime scan) timed on this
                 host over 64 sampled 2^24-even segments spread over [4, 1e12].
-  high_windows  64 windows above 1e15 (q by pre-sieve + Miller-Rabin) in one
+  high_windows  64 windows above 1e15 (q by pre-sieve + Baillie-PSW) in one
                 persistent launch, evens/s.
   partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
                 candidate pairs/s (the reference loop's p = 2..n/2 per n), with
@@ -175,7 +175,7 @@ def high_windows(count: int = 64, lo_evens: int = 10**6, hi_evens: int = 10**9, 
 
 
 def windows_leg(ctx) -> dict:
-    """High windows: q decided by pre-sieve + Miller-Rabin, all windows in one persistent launch."""
+    """High windows: q decided by pre-sieve + Baillie-PSW, all windows in one persistent launch."""
     wins = high_windows()
     evens = sum(e for _, e in wins)
     ctx.verify_windows([(a, min(e, 1 << 20)) for a, e in wins], P_SMALL)   # warm (tables, code)
@@ -282,7 +282,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--no-windows", action="store_true", help="skip the high-window (Miller-Rabin) leg")
+    ap.add_argument("--no-windows", action="store_true", help="skip the high-window (Baillie-PSW) leg")
     ap.add_argument("--no-counts", action="store_true", help="skip the partition-count leg")
     ap.add_argument("--no-big", action="store_true", help="skip the big_check (10^1000, 10^10000) leg")
     ap.add_argument("--copy-probe", action="store_true",

@@ -17,8 +17,10 @@
  *   - One context per host thread.  Calls on one context are serialised by
  *     the caller; contexts on one device may be used from different threads.
  *   - Even n, q and p are uint64; no step is approximated (sieves are exact
- *     and q beyond the sieved range is decided by deterministic Miller-Rabin
- *     with the 12 bases 2..37, valid for all 64-bit q).
+ *     and q beyond the sieved range is decided by tests that are exact for
+ *     every 64-bit q: Miller-Rabin with the prime bases 2, 3, 5, ... up to
+ *     the psi_k bound or with Sinclair's seven bases, or Baillie-PSW; all
+ *     agree with the reference's 12 bases 2..37).
  *
  * Bit layouts (identical to the reference)
  *   prime bitset  : bit i of the odd-only set <-> 2i+3, LSB-first in uint64
@@ -149,7 +151,7 @@ int gb_verify_range(gb_ctx *ctx, uint64_t start, uint64_t limit, uint64_t p_smal
                     const gb_tuning *tuning, gb_seg_stats *stats,
                     volatile uint64_t *progress_evens);
 
-/* ---- uneven high windows (Miller-Rabin for q) -------------------------- */
+/* ---- uneven high windows (q by pre-sieve and Baillie-PSW) -------------- */
 
 typedef struct gb_window {
     uint64_t start;   /* even >= 4 */
@@ -159,7 +161,8 @@ typedef struct gb_window {
 /* Every window in one persistent launch; per_window (nullable) gets one
  * gb_seg_stats per window, total (nullable) the sum.  Generalises the
  * reference's range loop to windows anywhere below 2^64, where q is decided
- * by a small-prime pre-sieve plus Miller-Rabin (goldbach.cu:160-179). */
+ * by a small-prime pre-sieve plus an exact 64-bit test in place of the
+ * reference's Miller-Rabin (goldbach.cu:160-179). */
 int gb_verify_windows(gb_ctx *ctx, const gb_window *windows, uint32_t count, uint64_t p_small,
                       uint32_t flags, gb_seg_stats *per_window, gb_seg_stats *total);
 
