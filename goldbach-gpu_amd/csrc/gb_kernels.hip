@@ -1662,12 +1662,12 @@ hipError_t launch_min_partition(hipStream_t s, const Tables& T, const uint64_t* 
 // against itself.  R[j] = P[M - j] (M the last index) turns it into
 // popcount(P[64w..] & R[64w + M - K ..]) over words w <= K/128.
 //
-// count_kernel: a workgroup takes 1024 even n in a block of 8192 (wave v,
-// lane l: n = base + 2(16g + v) + 128l), so every lane of a wave shifts R by
-// the same amount mod 64 (one funnel per word, no divergence), and a chunk of
-// CP_CHUNK words of P and the matching R words are staged in LDS once for all
-// 1024 n.  Per lane and word: two LDS reads (P broadcast, R sliding), a
-// funnel, an AND and a popcount.
+// count_kernel: workgroup (a, r) takes the 1024 n of index i = 65536 a + r + 64 t
+// (thread t, r < 64), so K = (n - 6)/2 is the same mod 64 for all of them and R is
+// shifted by one amount for the whole workgroup: per chunk of CP_CHUNK words of P,
+// the matching R words are staged in LDS already shifted (two v_alignbit_b32 per
+// word, once per workgroup).  Per thread and word: two LDS reads (P broadcast, R
+// at the thread's own offset, consecutive across lanes), an AND and a popcount.
 // ===========================================================================
 namespace {
 constexpr uint32_t CP_CHUNK = 2048;
@@ -1700,7 +1700,10 @@ __global__ void reverse_bits_kernel(const uint64_t* __restrict__ P, uint64_t M, 
     }
 }
 
-__global__ __launch_bounds__(1024) void count_kernel(const uint64_t* __restrict__ P, const uint64_t* __restrict__ R,
+// Narrow ranges (fewer than 65536 n): a workgroup takes 1024 consecutive even n in a
+// block of 8192 (wave v, lane l: n = base + 2(16g + v) + 128l); the n of a wave are 64
+// apart in K, so the wave shares one shift of R, applied per word in the loop.
+__global__ __launch_bounds__(1024) void count_kernel_narrow(const uint64_t* __restrict__ P, const uint64_t* __restrict__ R,
                                                      uint64_t M, uint64_t pwords, uint64_t start, uint64_t n_end,
                                                      uint64_t nchunks, uint64_t* __restrict__ counts) {
     __shared__ uint64_t sP[CP_CHUNK];
@@ -1765,6 +1768,82 @@ __global__ __launch_bounds__(1024) void count_kernel(const uint64_t* __restrict_
     if (valid && acc) atomicAdd((unsigned long long*)&counts[(n - start) / 2], (unsigned long long)acc);
 }
 
+// acc += popcount(x) in one v_bcnt_u32_b32 (the compiler pairs two counts into a v_add3_u32 instead)
+__device__ __forceinline__ void bcnt_acc(uint32_t& acc, uint32_t x) {
+    asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(x));
+}
+
+__global__ __launch_bounds__(1024) void count_kernel(const uint64_t* __restrict__ P, const uint64_t* __restrict__ R,
+                                                     uint64_t M, uint64_t pwords, uint64_t K0, uint64_t count,
+                                                     uint64_t nchunks, uint64_t* __restrict__ counts) {
+    __shared__ uint64_t sP[CP_CHUNK];
+    __shared__ uint64_t sR[CP_CHUNK + 1026];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t r = blockIdx.x & 63, a = blockIdx.x >> 6;
+    const uint64_t i_first = 65536 * a + r;                // this workgroup's n: i = i_first + 64 t
+    if (i_first >= count) return;                          // uniform
+    const uint64_t i = i_first + 64ull * tid;
+    const uint64_t i_last = i_first + 64ull * ((count - 1 - i_first) / 64 < 1023 ? (count - 1 - i_first) / 64 : 1023);
+    if ((int64_t)K0 + (int64_t)i_last < 0) return;        // uniform: only n = 4 here (the host adds it)
+    const bool valid = i < count;
+    // K = K0 + i with K0 = (start - 6)/2 (start = 4: K0 = -1, and i = 0 is n = 4, counted on the host)
+    const int64_t Ks = (int64_t)K0 + (int64_t)i;
+    const bool live = valid && Ks >= 0;
+    const uint64_t K = live ? (uint64_t)Ks : 0;
+    const uint64_t K_hi = (uint64_t)((int64_t)K0 + (int64_t)i_last);
+    const uint64_t s_min = M - K_hi, w_hi = (K_hi / 2) >> 6;
+    const uint32_t sh = (uint32_t)((M - K_hi) & 63);        // = (M - K) mod 64 for every thread here
+    const uint64_t s = M - K;
+    const uint32_t ro = live ? (uint32_t)((s >> 6) - (s_min >> 6)) : 0;   // <= 1024
+    // the largest ro here: the workgroup's first live n (K >= 0) has the smallest K
+    const int64_t K_first = (int64_t)K0 + (int64_t)i_first >= 0 ? (int64_t)K0 + (int64_t)i_first : (int64_t)K0 + (int64_t)i_first + 64;
+    const uint32_t ro_max = (uint32_t)(((M - (uint64_t)K_first) >> 6) - (s_min >> 6));
+    const uint64_t amax = K / 2;
+    const uint64_t wl = amax >> 6;
+    const uint64_t last_mask = (amax & 63) == 63 ? ~0ull : ((2ull << (amax & 63)) - 1);
+    uint32_t acc = 0;
+    for (uint64_t chunk = blockIdx.y; chunk < nchunks; chunk += gridDim.y) {
+        const uint64_t w0 = chunk * CP_CHUNK;
+        if (w0 > w_hi) break;                              // uniform
+        const uint64_t rb = w0 + (s_min >> 6);
+        for (uint32_t j = tid; j < CP_CHUNK; j += 1024) sP[j] = w0 + j < pwords ? P[w0 + j] : 0;
+        for (uint32_t j = tid; j < CP_CHUNK + ro_max + 2; j += 1024) {
+            const uint64_t x0 = rb + j < pwords ? R[rb + j] : 0, x1 = rb + j + 1 < pwords ? R[rb + j + 1] : 0;
+            sR[j] = funnel64(x0, x1, sh);
+        }
+        __syncthreads();
+        if (live && wl >= w0) {
+            const bool ends_here = wl - w0 < CP_CHUNK;
+            const uint32_t full = ends_here ? (uint32_t)(wl - w0) : CP_CHUNK;
+            const uint64_t* rr = sR + ro;
+            uint32_t j = 0;
+            for (; j + 4 <= full; j += 4) {                 // unrolled by hand: asm blocks unrolling
+                const uint64_t v0 = rr[j] & sP[j], v1 = rr[j + 1] & sP[j + 1];
+                const uint64_t v2 = rr[j + 2] & sP[j + 2], v3 = rr[j + 3] & sP[j + 3];
+                bcnt_acc(acc, (uint32_t)v0);
+                bcnt_acc(acc, (uint32_t)(v0 >> 32));
+                bcnt_acc(acc, (uint32_t)v1);
+                bcnt_acc(acc, (uint32_t)(v1 >> 32));
+                bcnt_acc(acc, (uint32_t)v2);
+                bcnt_acc(acc, (uint32_t)(v2 >> 32));
+                bcnt_acc(acc, (uint32_t)v3);
+                bcnt_acc(acc, (uint32_t)(v3 >> 32));
+            }
+            for (; j < full; ++j) {
+                const uint64_t v = rr[j] & sP[j];
+                bcnt_acc(acc, (uint32_t)v);
+                bcnt_acc(acc, (uint32_t)(v >> 32));
+            }
+            if (ends_here) {
+                const uint64_t v = rr[j] & sP[j] & last_mask;
+                acc += __builtin_popcount((uint32_t)v) + __builtin_popcount((uint32_t)(v >> 32));
+            }
+        }
+        __syncthreads();
+    }
+    if (live && acc) atomicAdd((unsigned long long*)&counts[i], (unsigned long long)acc);
+}
+
 hipError_t launch_count_partitions(hipStream_t s, const uint64_t* P, uint64_t* R, uint64_t M, uint64_t pwords,
                                    uint64_t start, uint64_t n_end, uint64_t* counts) {
     uint64_t blocks = (pwords + 255) / 256;
@@ -1772,14 +1851,24 @@ hipError_t launch_count_partitions(hipStream_t s, const uint64_t* P, uint64_t* R
     hipLaunchKernelGGL(reverse_bits_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, P, M, pwords, R);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const uint64_t evens = (n_end - start) / 2 + 1;
-    const uint64_t nb = (evens + 4095) / 4096;      // blocks of 8192 in n
-    const uint64_t w_top = (((n_end >= 6 ? n_end - 6 : 0) / 2) / 2 >> 6) + 1;
+    const uint64_t count = (n_end - start) / 2 + 1;
+    if (count < 65536) {
+        const uint64_t nb = (count + 4095) / 4096;              // blocks of 8192 in n
+        const uint64_t w_top = (((n_end >= 6 ? n_end - 6 : 0) / 2) / 2 >> 6) + 1;
+        const uint64_t nchunks = (w_top + CP_CHUNK - 1) / CP_CHUNK;
+        const uint32_t gy = (uint32_t)(nchunks < 65535 ? nchunks : 65535);
+        hipLaunchKernelGGL(count_kernel_narrow, dim3((uint32_t)(nb * 4), gy), dim3(1024), 0, s, P, R, M, pwords, start,
+                           n_end, nchunks, counts);
+        return hipGetLastError();
+    }
+    const uint64_t K0 = (start - 6) / 2;                   // wraps to 2^64 - 1 for start = 4 (K = -1)
+    const uint64_t nb = (count + 65535) / 65536 * 64;       // 64 residues of i mod 64 per 65536 n
+    const uint64_t w_top = ((n_end - 6) / 2 / 2 >> 6) + 1;
     const uint64_t nchunks = (w_top + CP_CHUNK - 1) / CP_CHUNK;
-    if (nb * 4 > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
     const uint32_t gy = (uint32_t)(nchunks < 65535 ? nchunks : 65535);
-    hipLaunchKernelGGL(count_kernel, dim3((uint32_t)(nb * 4), gy), dim3(1024), 0, s, P, R, M, pwords, start, n_end,
-                       nchunks, counts);
+    hipLaunchKernelGGL(count_kernel, dim3((uint32_t)nb, gy), dim3(1024), 0, s, P, R, M, pwords,
+                       start == 4 ? ~0ull : K0, count, nchunks, counts);
     return hipGetLastError();
 }
 

@@ -1,7 +1,8 @@
 """Partition counts c(n) on the device (gb_count_partitions) against the oracle
 and the reference's own goldbach_count (cpu_goldbach.cpp:29-36): ranges from
-n = 4 (2 + 2, 3 + 3), ranges crossing the kernel's blocks of 8192 n and its
-LDS chunks of 2048 words (n near 2^20 * k), ragged ends, published values
+n = 4 (2 + 2, 3 + 3), ranges crossing the narrow kernel's blocks of 8192 n and
+its LDS chunks of 2048 words (n near 2^20 * k), ranges of 65536 n and more (the
+kernel whose workgroups take n 128 apart), ragged ends, published values
 c(10^k) and argument errors."""
 import numpy as np
 import pytest
@@ -12,7 +13,10 @@ C_POW10 = (2, 6, 28, 127, 810, 5402, 38807, 291400, 2274205, 18200488)   # OEIS 
 
 
 @pytest.mark.parametrize("start,count", [(4, 1), (4, 2), (6, 1), (4, 5000), (8, 4097), (1_000_000, 3000),
-                                         (1_048_000, 700), (2_097_100, 130), (16_000_000, 64)])
+                                         (1_048_000, 700), (2_097_100, 130), (16_000_000, 64),
+                                         # 65536 n and more: workgroups of n 128 apart (one shift of R each)
+                                         (4, 65536), (6, 65536), (4, 70001), (123_456, 131_073),
+                                         (2_000_002, 65600), (3_000_000, 66000)])
 def test_ranges_vs_oracle(ctx, orc, start, count):
     got = ctx.count_partitions(start, count)
     ref = orc.count_partitions(start, count)
