@@ -20,8 +20,10 @@
 //     Newton doubling from the word inverse;
 //   * R mod q without a division: q's top bit t - 1 gives 2^(t-1) < q, and
 //     32 L - t + 1 modular doublings lift it to 2^(32 L);
-//   * base 2 multiplies by doubling; other bases by a Montgomery product with
-//     b R mod q (built from R mod q by doublings and additions).
+//   * base 2 multiplies by doubling, other bases b < 128 by one word-by-b
+//     product with its quotient by q read off the top words (no Montgomery
+//     product); b R mod q, the start value, comes from R mod q by doublings
+//     and additions.
 #include <hip/hip_runtime.h>
 
 #include "gb_big.hpp"
@@ -62,7 +64,7 @@ __host__ __device__ inline Layout layout(uint32_t L) {
     o.e = w; w += round4(L + 1);
     o.rq = w; w += round4(L);
     o.qmr = w; w += round4(L);
-    o.scan = w; w += 4 * NW_MAX + 8;        // wave totals, top carry, t, s, C
+    o.scan = w; w += 4 * NW_MAX + 8;        // wave totals, top carry, t, s, C, qhat
     o.total = w;
     return o;
 }
@@ -321,6 +323,87 @@ __device__ void add_mod(const Big& b, uint32_t* x, const uint32_t* a) {
     sub_if_ge(b, b.d, top, x);
 }
 
+// x = k x mod q for 2 < k < 128 (x < q; x in Montgomery form stays so): t = k x in L + 1
+// words, qhat = floor(t / q) from the top words (off by at most one), y = t - qhat q, and
+// one correction (add q when y < 0, subtract q when y >= q).  Three scans, O(L) work: the
+// multiply step of a strong test to a small base, in place of a Montgomery product.
+__device__ void mul_small_mod(const Big& b, uint32_t* x, uint32_t k) {
+    const uint32_t L = b.L, tid = threadIdx.x;
+    const uint32_t B = (L + bw() - 1) / bw();
+    const uint32_t nb = (L + B - 1) / B;
+    const uint32_t j0 = tid * B, j1 = j0 + B < L ? j0 + B : L;
+    // t = k x into d, the word above into t_top (carries below k < 2^7)
+    uint32_t c = 0, r0 = 0;
+    bool pr = true;
+    for (uint32_t j = j0; j < j1; ++j) {
+        const uint64_t v = (uint64_t)x[j] * k + c;
+        b.d[j] = (uint32_t)v;
+        c = (uint32_t)(v >> 32);
+        if (j == j0) r0 = (uint32_t)v;
+        else pr = pr && (uint32_t)v == 0xFFFFFFFFu;
+    }
+    CF f{c, INF};
+    if (j0 < j1 && pr && r0 >= 0xFFFFFF00u) f.th = 0u - r0;
+    uint32_t t_top;
+    uint32_t ci = scan_carry(f, nb, b.sc, &t_top);
+    for (uint32_t j = j0; j < j1 && ci; ++j) {
+        const uint64_t v = (uint64_t)b.d[j] + ci;
+        b.d[j] = (uint32_t)v;
+        ci = (uint32_t)(v >> 32);
+    }
+    __syncthreads();
+    if (tid == 0) {   // q >= 2^(32(L-1)) - 2^24, so its top two words carry the ratio to 2^-31
+        const double tt = (double)t_top * 18446744073709551616.0 + (double)b.d[L - 1] * 4294967296.0 + b.d[L - 2];
+        const double qq = (double)b.q[L - 1] * 4294967296.0 + b.q[L - 2] + b.q[L - 3] * (1.0 / 4294967296.0);
+        double r = floor(tt / qq);
+        r = r < 0.0 ? 0.0 : (r > (double)k ? (double)k : r);
+        b.sc[2 * NW_MAX + 5] = (uint32_t)r;
+    }
+    __syncthreads();
+    const uint32_t qh = b.sc[2 * NW_MAX + 5];
+    // y = t - qh q in place in d; a borrow-in bi below 2^8 adds a borrow out iff the block is below bi
+    uint32_t bo = 0;
+    bool allz = true;
+    for (uint32_t j = j0; j < j1; ++j) {
+        const uint64_t m = (uint64_t)qh * b.q[j] + bo;
+        const uint32_t lo = (uint32_t)m, dj = b.d[j], r = dj - lo;
+        b.d[j] = r;
+        bo = (uint32_t)(m >> 32) + (dj < lo ? 1u : 0u);
+        if (j == j0) r0 = r;
+        else allz = allz && r == 0;
+    }
+    CF g{bo, INF};
+    if (j0 < j1 && allz && r0 < 256u) g.th = r0 + 1;
+    uint32_t bfin;
+    uint32_t bi = scan_carry(g, nb, b.sc, &bfin);
+    for (uint32_t j = j0; j < j1 && bi; ++j) {
+        const uint32_t v = b.d[j];
+        b.d[j] = v - bi;
+        bi = v < bi ? 1u : 0u;
+    }
+    __syncthreads();
+    if (t_top < bfin) {                              // qh one too large: y + q, its carry out dropped
+        uint32_t cc = 0;
+        bool allf = true;
+        for (uint32_t j = j0; j < j1; ++j) {
+            const uint64_t v = (uint64_t)b.d[j] + b.q[j] + cc;
+            b.d[j] = (uint32_t)v;
+            cc = (uint32_t)(v >> 32);
+            allf = allf && (uint32_t)v == 0xFFFFFFFFu;
+        }
+        uint32_t unused;
+        uint32_t cj = scan_carry(CF{cc, (j0 < j1 && allf) ? 1u : INF}, nb, b.sc, &unused);
+        for (uint32_t j = j0; j < j1; ++j) {
+            const uint64_t v = (uint64_t)b.d[j] + cj;
+            x[j] = (uint32_t)v;
+            cj = (uint32_t)(v >> 32);
+        }
+        __syncthreads();
+    } else {
+        sub_if_ge(b, b.d, t_top - bfin, x);          // qh exact or one too small
+    }
+}
+
 // Montgomery reduction of the product T = b.t (2L words, T < q R): x = T / R mod q.
 // U = (T + M q) / R with M = (T mod R)(-1/q) mod R.  Of M q only the columns from L - 3 up
 // are summed: the low L words of T + M q are zero, so the carry C out of them is an integer
@@ -488,7 +571,7 @@ __global__ __launch_bounds__(BIG_BS) void big_mr_kernel(const uint32_t* __restri
         mont_sqr(b, b.x);
         if (ebit(b, (uint32_t)j)) {
             if (base == 2) dbl_mod(b, b.x);
-            else mont_mul(b, b.x, b.y);
+            else mul_small_mod(b, b.x, base);
         }
     }
     bool ok = equal(b, b.x, b.rq) || equal(b, b.x, b.qmr);
