@@ -83,20 +83,38 @@ __device__ inline CF compose(CF f, CF g) {
 
 __device__ inline uint32_t apply(CF f, uint32_t c) { return f.a + (c >= f.th ? 1u : 0u); }
 
+// The carry function of the lane that CTRL (a DPP control) names, or a marked one (a = MARK)
+// where that lane does not exist or ROW_MASK leaves the row out.
+constexpr uint32_t MARK = 0xFFFFFFFFu;   // no carry function has a this large
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ CF dpp_cf(CF f) {
+    CF g;
+    g.a = (uint32_t)__builtin_amdgcn_update_dpp((int)MARK, (int)f.a, CTRL, ROW_MASK, 0xF, false);
+    g.th = (uint32_t)__builtin_amdgcn_update_dpp((int)INF, (int)f.th, CTRL, ROW_MASK, 0xF, false);
+    return g;
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void scan_step(CF& inc) {
+    const CF g = dpp_cf<CTRL, ROW_MASK>(inc);
+    const CF c = compose(g, inc);
+    if (g.a != MARK) inc = c;
+}
+
 // Workgroup scan of the threads' carry functions over blocks 0..nb-1 (thread t = block t).
 // Returns the carry into this thread's block; *top gets the carry out of block nb-1.
+// Within a wave: DPP row shifts 1, 2, 4, 8, then row broadcasts 15 and 31 (no LDS).
 __device__ uint32_t scan_carry(CF f, uint32_t nb, uint32_t* sc, uint32_t* top) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     CF inc = f;
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-        CF g;
-        g.a = __shfl_up(inc.a, off, 64);
-        g.th = __shfl_up(inc.th, off, 64);
-        if (lane >= off) inc = compose(g, inc);
-    }
-    CF exc;
-    exc.a = __shfl_up(inc.a, 1, 64);
-    exc.th = __shfl_up(inc.th, 1, 64);
+    scan_step<0x111, 0xF>(inc);    // row_shr:1
+    scan_step<0x112, 0xF>(inc);    // row_shr:2
+    scan_step<0x114, 0xF>(inc);    // row_shr:4
+    scan_step<0x118, 0xF>(inc);    // row_shr:8
+    scan_step<0x142, 0xA>(inc);    // row_bcast:15 into rows 1 and 3
+    scan_step<0x143, 0xC>(inc);    // row_bcast:31 into rows 2 and 3
+    const CF exc = dpp_cf<0x138, 0xF>(inc);   // wave_shr:1: the lane before's inclusive function
     if (lane == 63) {
         sc[2 * wv] = inc.a;
         sc[2 * wv + 1] = inc.th;
