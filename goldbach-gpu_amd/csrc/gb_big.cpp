@@ -156,10 +156,14 @@ int gb_big_min_partition(gb_ctx* c, const uint32_t* n_words, uint32_t nwords, ui
     const uint64_t bad_words = p_max / 64 + 1;
     DevBuf<uint32_t> d_n, d_ell, d_r, d_bad, d_pass;
     DevBuf<gbk::BigTask> d_tasks;
-    // one batch = the tests the device holds at once (LDS and 2048 threads per CU)
+    // one batch = the tests the device holds at once (LDS and 2048 threads per CU); a
+    // 1024-thread test already fills its CU (two per CU took twice as long as one), so those
+    // run one per CU and the batches stay short
     const size_t lds = gbk::big_lds_bytes(L) + 256;
-    const uint32_t per_cu = std::max<uint32_t>(
-        1, std::min<uint32_t>((uint32_t)(gbk::LDS_PER_CU / lds), 2048 / gbk::big_threads(L)));
+    const uint32_t threads = gbk::big_threads(L);
+    const uint32_t per_cu = threads >= 1024 ? 1u
+                                            : std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)(gbk::LDS_PER_CU / lds),
+                                                                                       2048 / threads));
     const uint32_t batch = (uint32_t)std::max(1, c->cus) * per_cu;
     if (d_n.alloc(L) != hipSuccess || d_ell.alloc(ell.size()) != hipSuccess || d_r.alloc(ell.size()) != hipSuccess ||
         d_bad.alloc(bad_words) != hipSuccess || d_tasks.alloc(std::max<uint32_t>(batch, MAX_ROUNDS)) != hipSuccess ||
