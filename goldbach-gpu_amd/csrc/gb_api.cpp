@@ -202,7 +202,7 @@ int ensure_r0(gb_ctx* c, uint64_t n) {
 
 // Sieving-prime entries for a window whose first sieved q is rbase and whose
 // largest q is q_top: every prime any of its tiles may use.  A tile whose
-// sqrt(q) passes the table (2^22) only pre-sieves to MR_PRESIEVE and decides
+// sqrt(q) passes the table (2^22) pre-sieves with the whole table and decides
 // the survivors by Miller-Rabin.
 void set_residues(const gb_ctx* c, WinDesc* wd, uint64_t rbase, uint64_t q_top, uint64_t r0off) {
     const uint64_t s_top = sieve_prime_bound(q_top);

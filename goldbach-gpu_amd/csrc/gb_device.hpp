@@ -28,7 +28,7 @@ constexpr uint32_t CHUNK = 32;               // marks per medium-prime work item
 constexpr uint32_t WAVE_MARKS = 8;           // marks per lane of a wheel wave item (64 lanes: 512 marks)
 constexpr uint32_t CHUNK_SPAN = 48;          // ... spanning 48p bits: odd multiples p*m with 3 | m are
                                              // skipped (the pattern of 3 already cleared them)
-constexpr uint32_t MR_PRESIEVE = 1u << 18;   // pre-sieve bound for tiles decided by Miller-Rabin
+constexpr uint32_t MR_PRESIEVE = 1u << 22;   // pre-sieve bound of the tiles decided by Baillie-PSW (the whole table)
 
 struct Tables {
     const uint32_t* sp;     // odd primes <= SP_MAX ascending (sp[0] = 3, sp[17] = 67)

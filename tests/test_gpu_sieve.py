@@ -21,9 +21,10 @@ WINDOWS = [
     (3825123056546413051 - 40_000, 3825123056546413051 + 40_000),    # ... to bases 2..31
 ]
 
-# Base-2 strong pseudoprimes whose factors all exceed the 2^18 pre-sieve reach the
-# tiles' second test (the strong Lucas test of Baillie-PSW): 10670053 * 32010157,
-# 4195117 * 8390233, 262957 * 525913 * 788869 (a (6k+1)(12k+1)(18k+1) Carmichael number).
+# Base-2 strong pseudoprimes above 2^44: 10670053 * 32010157 and 4195117 * 8390233 have
+# every factor above the 2^22 pre-sieve, so they reach the tiles' second test (the strong
+# Lucas test of Baillie-PSW); the pre-sieve removes 262957 * 525913 * 788869 (a
+# (6k+1)(12k+1)(18k+1) Carmichael number).
 SPSP = [341550071728321, 35198009092261, 109094669922527929]
 # squares of primes above the pre-sieve: no Lucas parameter D exists for them
 SQUARES = [4194319**2, 2147483647**2]
