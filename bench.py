@@ -48,6 +48,8 @@ LIMIT = 10**12
 P_SMALL = 10**6
 PLAIN_SEG = 1 << 27
 HBM_PEAK_GBS = 8000.0
+CUS = 256                      # MI355X compute units
+CLOCK_HZ = 2.4e9               # MI355X peak engine clock
 ALG_BYTES_PER_EVEN = 0.375
 REF_PUBLISHED_EVENS_PER_S = 499_999_999_999 / 37.440   # RESULTS.md:48, 1x RTX 5090
 METRIC = "even n verified/s at N=1e12 on 1 GPU, batched vs plain; p99 query latency"
@@ -320,9 +322,14 @@ def main() -> int:
     achieved = ALG_BYTES_PER_EVEN * evens / (kms * 1e-3) / 1e9
 
     traffic = None
+    valu = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        if prof.get("sq_insts_valu_per_launch"):
+            # the kernel's real bound: VALU issue (wave-instructions per CU clock, peak 1)
+            valu = prof["sq_insts_valu_per_launch"] / (CUS * kms * 1e-3 * CLOCK_HZ)
 
     plain = None
     if not args.no_plain:
@@ -366,7 +373,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "wheel_kernel<512, 262144>", "kernel_ms": kms,
-                     "alg_bytes_per_launch": ALG_BYTES_PER_EVEN * evens},
+                     "alg_bytes_per_launch": ALG_BYTES_PER_EVEN * evens,
+                     "valu_issue_frac": valu,
+                     "valu_note": "SQ_INSTS_VALU per launch (rocprofv3 summary in profiles/) over 256 CUs x "
+                                  "the live kernel time x 2.4 GHz: the kernel is VALU-issue bound, not HBM bound"},
         "cpu_baseline": cpu,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
