@@ -5,7 +5,7 @@
 //                   word patterns for p <= 61, LDS atomics for 67 <= p,
 //                   medium primes split into 32-mark work items so a wave's
 //                   lanes stride p 32-bit words apart (distinct LDS banks),
-//                   Miller-Rabin on pre-sieve survivors above 2^44.
+//                   Baillie-PSW on pre-sieve survivors above 2^44.
 //   mark_words      Phase 1 + in-window Phase 2: a lane owns 64 consecutive
 //                   even n; for each odd prime p (wave-uniform) the 64 q = n-p
 //                   are 64 consecutive window bits, so one funnel shift + OR
@@ -307,7 +307,7 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     __syncthreads();
 
     // 4) above 2^44 the table cannot reach sqrt(q): survivors of the pre-sieve
-    //    are decided by deterministic Miller-Rabin.
+    //    are decided by Baillie-PSW (exact below 2^64).
     if (mr && any_valid) {
         mr_tile<BS>(win32, nbits, qv0 + 2 * g0 - 2ull * lead, tid, scratch, scratch_cap, wsum);
     }
