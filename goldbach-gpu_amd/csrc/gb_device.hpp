@@ -286,17 +286,6 @@ __device__ __forceinline__ uint64_t half_mod(uint64_t x, uint64_t n) {   // x / 
     return (x & 1) ? (x >> 1) + (n >> 1) + 1 : x >> 1;
 }
 
-// v (|v| < 2^31, |v| < n) in Montgomery form
-__device__ inline uint64_t mont_small(int64_t v, const Mont& M) {
-    const uint32_t a = (uint32_t)(v < 0 ? -v : v);
-    uint64_t am = 0;
-    for (int b = 31 - __clz(a); b >= 0; --b) {
-        am = add_mod(am, am, M.n);
-        if ((a >> b) & 1) am = add_mod(am, M.one, M.n);
-    }
-    return (v < 0 && am) ? M.n - am : am;
-}
-
 // Jacobi symbol (a/m), m odd > 0
 __device__ inline int jacobi32(uint32_t a, uint32_t m) {
     int t = 1;
@@ -315,10 +304,36 @@ __device__ inline int jacobi32(uint32_t a, uint32_t m) {
     return m == 1 ? t : 0;
 }
 
+// x mod c for c < 2^16, with c32 = 2^32 mod c
+__device__ __forceinline__ uint32_t mod_u64_small(uint64_t x, uint32_t c, uint32_t c32) {
+    return (((uint32_t)(x >> 32) % c) * c32 + (uint32_t)x % c) % c;
+}
+
+// a^{-1} mod m for odd m > 1 by the extended Euclidean algorithm; 0 when gcd(a, m) > 1
+__device__ inline uint32_t inv_mod32(uint32_t a, uint32_t m) {
+    int32_t r0 = (int32_t)m, r1 = (int32_t)a, x0 = 0, x1 = 1;
+    while (r1) {
+        const int32_t t = r0 / r1, r2 = r0 - t * r1, x2 = x0 - t * x1;
+        r0 = r1; r1 = r2; x0 = x1; x1 = x2;
+    }
+    if (r0 != 1) return 0;
+    return (uint32_t)(x0 < 0 ? x0 + (int32_t)m : x0);
+}
+
 // Strong Lucas probable-prime test of odd n > 2^32 (a perfect square, for which no D
 // exists, is composite).  Returns 1 (passes), 0 (composite) or -1 when no D below 2^16
 // turned up (the first D is small for every non-square; the caller then decides by
 // Miller-Rabin).
+//
+// The ladder runs on W_k = V_{2k} / Q^k, the Lucas V sequence with parameters (P', 1),
+// P' = P^2/Q - 2 = 1/Q - 2 mod n (the roots of x^2 - x + Q squared and divided by Q have
+// product 1 and sum P').  With n + 1 = d 2^s, d odd, m = (d - 1)/2, and D, Q units mod n:
+//   U_d = 0  <=>  W_{m+1} = W_m        (D U_d = 2 V_{d+1} - V_d,  V_d = V_{d+1} + Q V_{d-1})
+//   V_d = 0  <=>  W_{m+1} + W_m = 0
+//   V_{d 2^r} = 0  <=>  W_{d 2^{r-1}} = 0   for 1 <= r < s.
+// A step of a Q = 1 sequence, (W_k, W_{k+1}) -> (W_{2k}, W_{2k+1}) or (W_{2k+1}, W_{2k+2}),
+// is one product and one square whichever the bit (W_{2k} = W_k^2 - 2, W_{2k+1} = W_k W_{k+1}
+// - P'), so lanes whose bits differ do not diverge.
 __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
     const uint64_t r = isqrt64(n);
     if (r * r == n) return 0;
@@ -328,35 +343,55 @@ __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
         if (ad >= 65536) return -1;
         // (D/n) = (n mod |D| / |D|) by reciprocity, times (-1/n) for negative D
         const uint32_t c32 = (0xFFFFFFFFu % ad + 1) % ad;                       // 2^32 mod |D|
-        const uint32_t nm = (((uint32_t)(n >> 32) % ad) * c32 + (uint32_t)n % ad) % ad;
-        int j = jacobi32(nm, ad);
+        int j = jacobi32(mod_u64_small(n, ad, c32), ad);
         if ((((ad - 1) >> 1) & 1) && (((n - 1) >> 1) & 1)) j = -j;
         if (D < 0 && (n & 3) == 3) j = -j;
         if (j == -1) break;
         if (j == 0) return 0;                                                   // |D| < n shares a factor
         D = D > 0 ? -D - 2 : -D + 2;
     }
-    const uint64_t Dm = mont_small(D, M), Qm = mont_small((1 - D) / 4, M);
+    // 1/Q in Montgomery form: halve R mod n for the power of two in |Q|, then divide by the
+    // odd part c exactly, (x + t n)/c with x + t n = 0 mod c, as a product with c^{-1} mod 2^64
+    const int64_t Q = (1 - D) / 4;
+    const uint32_t aq = (uint32_t)(Q < 0 ? -Q : Q);
+    const uint32_t e = __builtin_ctz(aq), c = aq >> e;
+    uint64_t qi = M.one;
+    for (uint32_t i = 0; i < e; ++i) qi = half_mod(qi, M.n);
+    if (c > 1) {
+        const uint32_t c32 = (0xFFFFFFFFu % c + 1) % c;
+        const uint32_t inv = inv_mod32(mod_u64_small(n, c, c32), c);
+        if (!inv) return 0;                                                     // Q shares a factor with n
+        const uint32_t t = (c - mod_u64_small(qi, c, c32) * inv % c) % c;
+        uint64_t ci = c;                                                        // c^{-1} mod 2^64
+        for (int i = 0; i < 5; ++i) ci *= 2 - (uint64_t)c * ci;
+        qi = (qi + (uint64_t)t * M.n) * ci;
+    }
+    if (Q < 0 && qi) qi = M.n - qi;
+    const uint64_t two = add_mod(M.one, M.one, M.n);
+    const uint64_t pp = sub_mod(qi, two, M.n);                                  // P'
     uint64_t d = n + 1;
     const uint32_t s = __builtin_ctzll(d);
-    d >>= s;
-    uint64_t U = M.one, V = M.one, Qk = Qm;                                      // k = 1
-    for (int b = 62 - __clzll((long long)d); b >= 0; --b) {
-        U = mont_mul(U, V, M);                                                   // k -> 2k
-        V = sub_mod(mont_mul(V, V, M), add_mod(Qk, Qk, M.n), M.n);
-        Qk = mont_mul(Qk, Qk, M);
-        if ((d >> b) & 1) {                                                      // 2k -> 2k + 1
-            const uint64_t u1 = half_mod(add_mod(U, V, M.n), M.n);
-            V = half_mod(add_mod(mont_mul(Dm, U, M), V, M.n), M.n);
-            U = u1;
-            Qk = mont_mul(Qk, Qm, M);
-        }
+    const uint64_t m = d >> (s + 1);
+    uint64_t w0 = two, w1 = pp;                                                 // W_0, W_1
+    int b = -1;
+    if (m) {                                                                    // top bit: W_1, W_2
+        w0 = pp;
+        w1 = sub_mod(mont_mul(pp, pp, M), two, M.n);
+        b = 62 - __clzll((long long)m);
     }
-    if (U == 0 || V == 0) return 1;
+    for (; b >= 0; --b) {
+        const bool bit = (m >> b) & 1;
+        const uint64_t x = sub_mod(mont_mul(w0, w1, M), pp, M.n);              // W_{2k+1}
+        const uint64_t h = bit ? w1 : w0;
+        const uint64_t y = sub_mod(mont_mul(h, h, M), two, M.n);               // W_{2k+2} or W_{2k}
+        w0 = bit ? x : y;
+        w1 = bit ? y : x;
+    }
+    if (w0 == w1 || add_mod(w0, w1, M.n) == 0) return 1;                       // U_d = 0 or V_d = 0
+    uint64_t w = sub_mod(mont_mul(w0, w1, M), pp, M.n);                         // W_d
     for (uint32_t i = 1; i < s; ++i) {
-        V = sub_mod(mont_mul(V, V, M), add_mod(Qk, Qk, M.n), M.n);
-        if (V == 0) return 1;
-        Qk = mont_mul(Qk, Qk, M);
+        if (w == 0) return 1;                                                   // V_{d 2^i} = 0
+        w = sub_mod(mont_mul(w, w, M), two, M.n);
     }
     return 0;
 }

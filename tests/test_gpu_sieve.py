@@ -17,6 +17,7 @@ WINDOWS = [
     (17_592_177_655_809, 17_592_186_244_417),       # straddles 2^44: some tiles sieve, some test
     (10**18 + 1, 10**18 + 100_001),
     (2**64 - 2**21 + 1, 2**64 - 1),                 # top of the 64-bit range
+    (2**61 - 1 - 100_000, 2**61 - 1 + 100_000),     # Mersenne prime: n + 1 = 2^61, the Lucas ladder is empty
     (341550071728321 - 100_000, 341550071728321 + 100_000),          # strong pseudoprime to bases 2..19
     (3825123056546413051 - 40_000, 3825123056546413051 + 40_000),    # ... to bases 2..31
 ]
