@@ -168,13 +168,39 @@ struct Mont {
     uint64_t one;   // R mod n
 };
 
-__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, const Mont& M) {
-    uint64_t lo = a * b;
-    uint64_t hi = __umul64hi(a, b);
-    uint64_t m = lo * M.ninv;
-    uint64_t mh = __umul64hi(m, M.n);
-    uint64_t t = hi - mh;
+// The products are written on 32-bit limbs so that each is one v_mad_u64_u32 and the low
+// half of ab is formed once: m = lo(ab) * n^{-1} takes three word products and the high half
+// of mn four, since its low half equals lo(ab) (left to itself the compiler rebuilds lo(ab)
+// inside m = a * (b * n^{-1})).
+__device__ __forceinline__ uint64_t mont_redc(uint64_t hi, uint32_t t0, uint32_t t1, const Mont& M) {
+    const uint32_t n0 = (uint32_t)M.n, n1 = (uint32_t)(M.n >> 32);
+    const uint32_t i0 = (uint32_t)M.ninv, i1 = (uint32_t)(M.ninv >> 32);
+    const uint64_t q = (uint64_t)t0 * i0;
+    const uint32_t m0 = (uint32_t)q, m1 = (uint32_t)(q >> 32) + t0 * i1 + t1 * i0;
+    const uint64_t u = (uint64_t)m1 * n0 + __umulhi(m0, n0);     // the low word of m0 n0 is t0
+    const uint64_t v = (uint64_t)m0 * n1 + (uint32_t)u;
+    const uint64_t mh = (uint64_t)m1 * n1 + (u >> 32) + (v >> 32);
+    const uint64_t t = hi - mh;
     return hi < mh ? t + M.n : t;
+}
+
+__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, const Mont& M) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0;
+    const uint64_t x = (uint64_t)a1 * b0 + (p00 >> 32);
+    const uint64_t y = (uint64_t)a0 * b1 + (uint32_t)x;
+    const uint64_t hi = (uint64_t)a1 * b1 + (x >> 32) + (y >> 32);
+    return mont_redc(hi, (uint32_t)p00, (uint32_t)y, M);
+}
+
+// a^2: the cross product a0 a1 once, doubled
+__device__ __forceinline__ uint64_t mont_sqr(uint64_t a, const Mont& M) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint64_t p00 = (uint64_t)a0 * a0;
+    const uint64_t c = (uint64_t)a0 * a1;
+    const uint64_t y = (uint64_t)(uint32_t)(c << 1) + (p00 >> 32);
+    const uint64_t hi = (uint64_t)a1 * a1 + (c >> 31) + (y >> 32);
+    return mont_redc(hi, (uint32_t)p00, (uint32_t)y, M);
 }
 
 __device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t n) {
@@ -204,13 +230,13 @@ __device__ __forceinline__ bool sprp(uint64_t a, uint64_t d, uint32_t r, const M
     uint64_t x = am;
     int top = 63 - __clzll((long long)d);
     for (int b = top - 1; b >= 0; --b) {
-        x = mont_mul(x, x, M);
+        x = mont_sqr(x, M);
         if ((d >> b) & 1) x = mont_mul(x, am, M);
     }
     const uint64_t minus_one = M.n - M.one;
     if (x == M.one || x == minus_one) return true;
     for (uint32_t i = 1; i < r; ++i) {
-        x = mont_mul(x, x, M);
+        x = mont_sqr(x, M);
         if (x == minus_one) return true;
         if (x == M.one) return false;
     }
@@ -222,13 +248,13 @@ __device__ __forceinline__ bool sprp2(uint64_t d, uint32_t r, const Mont& M) {
     uint64_t x = add_mod(M.one, M.one, M.n);   // 2 in Montgomery form
     const int top = 63 - __clzll((long long)d);
     for (int b = top - 1; b >= 0; --b) {
-        x = mont_mul(x, x, M);
+        x = mont_sqr(x, M);
         if ((d >> b) & 1) x = add_mod(x, x, M.n);
     }
     const uint64_t minus_one = M.n - M.one;
     if (x == M.one || x == minus_one) return true;
     for (uint32_t i = 1; i < r; ++i) {
-        x = mont_mul(x, x, M);
+        x = mont_sqr(x, M);
         if (x == minus_one) return true;
         if (x == M.one) return false;
     }
@@ -376,14 +402,14 @@ __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
     int b = -1;
     if (m) {                                                                    // top bit: W_1, W_2
         w0 = pp;
-        w1 = sub_mod(mont_mul(pp, pp, M), two, M.n);
+        w1 = sub_mod(mont_sqr(pp, M), two, M.n);
         b = 62 - __clzll((long long)m);
     }
     for (; b >= 0; --b) {
         const bool bit = (m >> b) & 1;
         const uint64_t x = sub_mod(mont_mul(w0, w1, M), pp, M.n);              // W_{2k+1}
         const uint64_t h = bit ? w1 : w0;
-        const uint64_t y = sub_mod(mont_mul(h, h, M), two, M.n);               // W_{2k+2} or W_{2k}
+        const uint64_t y = sub_mod(mont_sqr(h, M), two, M.n);               // W_{2k+2} or W_{2k}
         w0 = bit ? x : y;
         w1 = bit ? y : x;
     }
@@ -391,7 +417,7 @@ __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
     uint64_t w = sub_mod(mont_mul(w0, w1, M), pp, M.n);                         // W_d
     for (uint32_t i = 1; i < s; ++i) {
         if (w == 0) return 1;                                                   // V_{d 2^i} = 0
-        w = sub_mod(mont_mul(w, w, M), two, M.n);
+        w = sub_mod(mont_sqr(w, M), two, M.n);
     }
     return 0;
 }
