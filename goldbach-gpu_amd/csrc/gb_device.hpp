@@ -209,13 +209,25 @@ __device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t n) 
     return (s < a || s >= n) ? s - n : s;
 }
 
+// 2^64 mod n for odd n > 1.  From 2^40 up the quotient Q = floor(2^64 / n) is below 2^24,
+// so the rounded fp64 quotient q0 is Q - 1, Q or Q + 1 (relative error under 2^-52); with
+// q = max(q0, 2) - 1 <= Q the remainder 2^64 - q n is exact in 64 bits and below 3n.
+__device__ __forceinline__ uint64_t pow64_mod(uint64_t n) {
+    if (n < (1ull << 40)) return (0ull - n) % n;
+    if (n > (1ull << 63)) return 0ull - n;
+    const uint64_t q0 = (uint64_t)(18446744073709551616.0 / (double)n);
+    uint64_t r = 0ull - (max(q0, (uint64_t)2) - 1) * n;
+    r = r >= n ? r - n : r;
+    return r >= n ? r - n : r;
+}
+
 __device__ __forceinline__ Mont mont_setup(uint64_t n) {
     Mont M;
     M.n = n;
     uint64_t x = n;                    // n*n == 1 mod 8 for odd n: 3 correct bits
     for (int i = 0; i < 5; ++i) x *= 2 - n * x;   // 6, 12, 24, 48, 96 bits
     M.ninv = x;
-    M.one = (0ull - n) % n;            // 2^64 mod n
+    M.one = pow64_mod(n);              // 2^64 mod n
     return M;
 }
 
@@ -346,10 +358,67 @@ __device__ inline uint32_t inv_mod32(uint32_t a, uint32_t m) {
     return (uint32_t)(x0 < 0 ? x0 + (int32_t)m : x0);
 }
 
+// Jacobi symbols (a/m) of a table of small odd m as bit masks over a < m, built by the
+// compiler from jacobi_c (the same algorithm as jacobi32).
+__host__ __device__ constexpr int jacobi_c(uint32_t a, uint32_t m) {
+    int t = 1;
+    a %= m;
+    while (a) {
+        while (!(a & 1)) {
+            a >>= 1;
+            if ((m & 7) == 3 || (m & 7) == 5) t = -t;
+        }
+        const uint32_t x = a;
+        a = m;
+        m = x;
+        if ((a & 3) == 3 && (m & 3) == 3) t = -t;
+        a %= m;
+    }
+    return m == 1 ? t : 0;
+}
+
+__host__ __device__ constexpr uint64_t jacobi_mask(uint32_t m, int v) {
+    uint64_t s = 0;
+    for (uint32_t a = 0; a < m; ++a)
+        if (jacobi_c(a, m) == v) s |= 1ull << a;
+    return s;
+}
+
+// Selfridge's candidates D = 5, -7, 9, ..., -47 (|D| <= 47).  Each is 1 mod 4, so by
+// reciprocity (D/n) = (n mod |D| / |D|) for every odd n; n mod |D| comes from n modulo one
+// of 218243025 = 3^3 5^2 7 11 13 17 19, 31367009 = 23 29 31 37 41 and 2021 = 43 47.
+struct SelfridgeD {
+    int32_t D;
+    uint32_t m, group;
+    uint64_t minus, zero;   // a with (a/m) = -1, with (a/m) = 0
+};
+__host__ __device__ constexpr SelfridgeD selfridge_d(int32_t D, uint32_t group) {
+    const uint32_t m = (uint32_t)(D < 0 ? -D : D);
+    return SelfridgeD{D, m, group, jacobi_mask(m, -1), jacobi_mask(m, 0)};
+}
+constexpr int SELFRIDGE_N = 22;
+constexpr SelfridgeD SELFRIDGE[SELFRIDGE_N] = {
+    selfridge_d(5, 0),   selfridge_d(-7, 0),  selfridge_d(9, 0),   selfridge_d(-11, 0), selfridge_d(13, 0),
+    selfridge_d(-15, 0), selfridge_d(17, 0),  selfridge_d(-19, 0), selfridge_d(21, 0),  selfridge_d(-23, 1),
+    selfridge_d(25, 0),  selfridge_d(-27, 0), selfridge_d(29, 1),  selfridge_d(-31, 1), selfridge_d(33, 0),
+    selfridge_d(-35, 0), selfridge_d(37, 1),  selfridge_d(-39, 0), selfridge_d(41, 1),  selfridge_d(-43, 2),
+    selfridge_d(45, 0),  selfridge_d(-47, 2)};
+static_assert(SELFRIDGE[0].minus == 0xC && SELFRIDGE[0].zero == 0x1, "(a/5) = -1 for a = 2, 3");
+static_assert(SELFRIDGE[1].minus == 0x68 && SELFRIDGE[2].minus == 0 && SELFRIDGE[2].zero == 0x49, "(a/7), (a/9)");
+static_assert(SELFRIDGE[5].minus == 0x6880 && SELFRIDGE[5].zero == 0x1669, "(a/15)");
+static_assert(218243025u == 27u * 25u * 7u * 11u * 13u * 17u * 19u && 31367009u == 23u * 29u * 31u * 37u * 41u &&
+              2021u == 43u * 47u, "residue groups");
+
 // Strong Lucas probable-prime test of odd n > 2^32 (a perfect square, for which no D
 // exists, is composite).  Returns 1 (passes), 0 (composite) or -1 when no D below 2^16
 // turned up (the first D is small for every non-square; the caller then decides by
 // Miller-Rabin).
+//
+// D: the table above settles nearly every n (a lane needs 2.2 candidates on average, a wave
+// of 64 primes about 9 and rarely more than 20): a candidate costs a remainder by a constant
+// and two mask tests, and a wave leaves the table once every lane is decided.  Past |D| = 47
+// the search goes on with jacobi32; a square never meets (D/n) = -1, so it is only looked
+// for there.
 //
 // The ladder runs on W_k = V_{2k} / Q^k, the Lucas V sequence with parameters (P', 1),
 // P' = P^2/Q - 2 = 1/Q - 2 mod n (the roots of x^2 - x + Q squared and divided by Q have
@@ -361,20 +430,29 @@ __device__ inline uint32_t inv_mod32(uint32_t a, uint32_t m) {
 // is one product and one square whichever the bit (W_{2k} = W_k^2 - 2, W_{2k+1} = W_k W_{k+1}
 // - P'), so lanes whose bits differ do not diverge.
 __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
-    const uint64_t r = isqrt64(n);
-    if (r * r == n) return 0;
-    int64_t D = 5;
-    for (;;) {
-        const uint32_t ad = (uint32_t)(D < 0 ? -D : D);
-        if (ad >= 65536) return -1;
-        // (D/n) = (n mod |D| / |D|) by reciprocity, times (-1/n) for negative D
-        const uint32_t c32 = (0xFFFFFFFFu % ad + 1) % ad;                       // 2^32 mod |D|
-        int j = jacobi32(mod_u64_small(n, ad, c32), ad);
-        if ((((ad - 1) >> 1) & 1) && (((n - 1) >> 1) & 1)) j = -j;
-        if (D < 0 && (n & 3) == 3) j = -j;
-        if (j == -1) break;
-        if (j == 0) return 0;                                                   // |D| < n shares a factor
-        D = D > 0 ? -D - 2 : -D + 2;
+    const uint32_t res[3] = {(uint32_t)(n % 218243025ull), (uint32_t)(n % 31367009ull), (uint32_t)(n % 2021ull)};
+    int64_t D = 0;                                                              // 0: undecided
+#pragma unroll
+    for (int k = 0; k < SELFRIDGE_N; ++k) {
+        if (D == 0) {
+            const uint32_t a = res[SELFRIDGE[k].group] % SELFRIDGE[k].m;
+            if ((SELFRIDGE[k].zero >> a) & 1) return 0;                          // |D| < n shares a factor
+            if ((SELFRIDGE[k].minus >> a) & 1) D = SELFRIDGE[k].D;
+        }
+    }
+    if (D == 0) {
+        const uint64_t r = isqrt64(n);
+        if (r * r == n) return 0;
+        D = 49;
+        for (;;) {
+            const uint32_t ad = (uint32_t)(D < 0 ? -D : D);
+            if (ad >= 65536) return -1;
+            const uint32_t c32 = (0xFFFFFFFFu % ad + 1) % ad;                   // 2^32 mod |D|
+            const int j = jacobi32(mod_u64_small(n, ad, c32), ad);             // = (D/n): D = 1 mod 4
+            if (j == -1) break;
+            if (j == 0) return 0;
+            D = D > 0 ? -D - 2 : -D + 2;
+        }
     }
     // 1/Q in Montgomery form: halve R mod n for the power of two in |Q|, then divide by the
     // odd part c exactly, (x + t n)/c with x + t n = 0 mod c, as a product with c^{-1} mod 2^64

@@ -40,6 +40,20 @@ def test_strong_pseudoprime_is_cleared(ctx, orc, q):
     assert np.array_equal(ctx.sieve_window(q - 20_000, q + 20_000), orc.window_bits(q - 20_000, q + 20_000))
 
 
+# Primes 1 + k * 47#/2 (k = 28, 38): 1 modulo every odd prime up to 47, so (D/q) = 1 for
+# every Selfridge candidate with |D| <= 47 and the strong Lucas test has to search on
+# (to D = 53 and D = -59).
+LATE_D = [8608456956238879741, 11682905869181336791]
+
+
+@pytest.mark.parametrize("q", LATE_D)
+def test_lucas_parameter_beyond_47(ctx, orc, q):
+    assert orc.is_prime(q)
+    bits = ctx.sieve_window(q - 2 * 64, q + 2 * 64)
+    assert unpack(bits, 129)[64] == 1
+    assert np.array_equal(ctx.sieve_window(q - 40_000, q + 40_000), orc.window_bits(q - 40_000, q + 40_000))
+
+
 @pytest.mark.parametrize("q_low,q_high", WINDOWS)
 def test_window_matches_oracle(ctx, orc, q_low, q_high):
     got = ctx.sieve_window(q_low, q_high)
