@@ -35,6 +35,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -54,6 +55,9 @@ ALG_BYTES_PER_EVEN = 0.375
 REF_PUBLISHED_EVENS_PER_S = 499_999_999_999 / 37.440   # RESULTS.md:48, 1x RTX 5090
 METRIC = "even n verified/s at N=1e12 on 1 GPU, batched vs plain; p99 query latency"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "wheel_kernel_1e12_traffic.json")
+WINDOWS_PROFILE = os.path.join(ROOT, "profiles", "high_windows_mr.json")
+PRESIEVE = 1 << 22             # the high-window tiles pre-sieve with every table prime below 2^22
+SQR_VALU = 28                  # VALU instructions of one 64-bit Montgomery square (mont_sqr, gfx950)
 
 
 def device_sync() -> None:
@@ -176,6 +180,35 @@ def high_windows(count: int = 64, lo_evens: int = 10**6, hi_evens: int = 10**9, 
     return [(int(a / 2) * 2, int(e)) for a, e in zip(starts, lens)]
 
 
+def presieve_density() -> float:
+    """Fraction of odd q with no prime factor in [3, 2^22): the product of (1 - 1/p) over the table."""
+    s = np.ones(PRESIEVE, dtype=bool)
+    s[:2] = False
+    for i in range(2, int(PRESIEVE ** 0.5) + 1):
+        if s[i]:
+            s[i * i::i] = False
+    ps = np.nonzero(s)[0][1:].astype(np.float64)          # odd primes
+    return float(np.prod(1.0 - 1.0 / ps))
+
+
+def montgomery_products(wins) -> float:
+    """Model of the 64-bit Montgomery products the windows need (one product = one mulmod).
+
+    A window of e evens near q needs the primality of about e odd q; e * S of them survive the
+    pre-sieve (S = presieve_density()) and take the base-2 strong test, b - 2 products for q of
+    b bits (b - 3 squarings of the ladder over d, n - 1 = d 2^r with E[r] = 2, and E[r - 1] = 1
+    more); the e * 2/ln q primes among them also take the strong Lucas test, 2b - 7 products
+    (two per bit of m = (d - 1)/2, n + 1 = d 2^s, one for the top bit, one for W_d, E[s - 1] = 1).
+    """
+    dens = presieve_density()
+    total = 0.0
+    for a, e in wins:
+        q = a + e
+        b = q.bit_length()
+        total += e * dens * (b - 2) + e * 2.0 / math.log(q) * (2 * b - 7)
+    return total
+
+
 def windows_leg(ctx) -> dict:
     """High windows: q decided by pre-sieve + Baillie-PSW, all windows in one persistent launch."""
     wins = high_windows()
@@ -187,9 +220,22 @@ def windows_leg(ctx) -> dict:
     device_sync()
     dt = time.perf_counter() - t0
     assert tot["evens"] == evens and tot["failures"] == 0
+    products = montgomery_products(wins)
+    ksec = tot["kernel_ms"] * 1e-3
+    valu = None
+    if os.path.exists(WINDOWS_PROFILE):
+        with open(WINDOWS_PROFILE) as f:
+            prof = json.load(f)
+        if prof.get("sq_insts_valu_per_launch"):
+            valu = prof["sq_insts_valu_per_launch"] / (CUS * ksec * CLOCK_HZ)
     return {"workload": "64 windows, uniform 1e6..1e9 evens, starts log-uniform in [1e15, 1.8e19] (seed 0)",
             "evens": evens, "value": evens / dt, "unit": "evens/s", "seconds": dt,
-            "kernel_ms": tot["kernel_ms"], "phase2": tot["phase2"]}
+            "kernel_ms": tot["kernel_ms"], "phase2": tot["phase2"],
+            "roofline": {"bound": "valu", "unit": "Montgomery products/s", "achieved": products / ksec,
+                         "products": products,
+                         "peak": CUS * CLOCK_HZ * 64 / SQR_VALU, "frac": products / ksec / (CUS * CLOCK_HZ * 64 / SQR_VALU),
+                         "peak_note": "every VALU slot of 256 CUs at 2.4 GHz spent on 28-instruction squarings",
+                         "valu_issue_frac": valu}}
 
 
 COUNT_LIMIT = 3 * 10**7
