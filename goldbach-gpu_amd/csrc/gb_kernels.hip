@@ -117,7 +117,7 @@ __device__ uint32_t block_scan(uint32_t c, uint32_t tid, uint32_t* __restrict__ 
 //   pass 1: every survivor left gets the strong Lucas test;
 // a failed test clears the bit.  No composite below 2^64 passes both, so the answer is
 // the one of the Miller-Rabin bases 2..37; nearly every base-2 survivor is prime, and
-// one Lucas test costs about as much as three more strong tests.
+// one Lucas test (a product and a square per bit) costs about two base-2 tests.
 template <int BS>
 __device__ void mr_tile(uint32_t* __restrict__ win32, uint32_t nbits, uint64_t qb, uint32_t tid,
                         uint32_t* __restrict__ queue, uint32_t qcap, uint32_t* __restrict__ wsum) {
