@@ -14,6 +14,11 @@ The JSON line carries:
                 its average launch time, measured with HIP events on the
                 library's stream around each launch; traffic from the
                 committed rocprofv3 counter summary when present.
+  plain         the plain form over [4, 1e12]; beside it [4, 1e11] in 2^27-even
+                segments at P_SMALL = 1e6 and 4e9 (the cost must not grow with
+                P_SMALL) and [4, 1e9] in both forms, each against the reference's
+                published RTX 5090 time; --ref-cpu adds the reference's own
+                cpu_goldbach 1000000000 (one thread) built into oracle/_ref.
   cpu_baseline  the CPU restatement (oracle/, OpenMP over segments, the
                 reference algorithm: per-n ascending prime scan) timed on this
                 host over 64 sampled 2^24-even segments spread over [4, 1e12].
@@ -37,6 +42,7 @@ import ctypes
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -56,6 +62,8 @@ REF_PUBLISHED_EVENS_PER_S = 499_999_999_999 / 37.440   # RESULTS.md:48, 1x RTX 5
 METRIC = "even n verified/s at N=1e12 on 1 GPU, batched vs plain; p99 query latency"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "wheel_kernel_1e12_traffic.json")
 WINDOWS_PROFILE = os.path.join(ROOT, "profiles", "high_windows_mr.json")
+REF_CPU_GOLDBACH = os.path.join(ROOT, "oracle", "_ref", "cpu_goldbach")   # the reference's CPU program, built from its sources
+REF_GPU_SECONDS = {10**9: 0.141018, 10**11: 3.3115}   # RESULTS.md:45,47, 1x RTX 5090
 PRESIEVE = 1 << 22             # the high-window tiles pre-sieve with every table prime below 2^22
 SQR_VALU = 28                  # VALU instructions of one 64-bit Montgomery square (mont_sqr, gfx950)
 
@@ -178,6 +186,56 @@ def high_windows(count: int = 64, lo_evens: int = 10**6, hi_evens: int = 10**9, 
     starts = np.exp(rng.uniform(np.log(1e15), np.log(1.8e19), size=count))
     lens = rng.integers(lo_evens, hi_evens + 1, size=count)
     return [(int(a / 2) * 2, int(e)) for a, e in zip(starts, lens)]
+
+
+def plain_1e11(ctx, gb) -> dict:
+    """The plain form on [4, 1e11] in 2^27-even segments (the reference's published setting,
+    goldbach 100000000000 --seg-size=134217728) at P_SMALL = 1e6 and at the allowed maximum 4e9:
+    the answers agree and the cost does not grow with P_SMALL."""
+    limit, evens = 10**11, (10**11 - 4) // 2 + 1
+    out = {"workload": "all even n in [4, 1e11], plain form, 2^27 evens per segment",
+           "reference_gpu_seconds": REF_GPU_SECONDS[limit]}
+    ctx.verify_range(4, limit, P_SMALL, form=gb.GB_FORM_PLAIN, seg_evens=PLAIN_SEG)   # warm
+    ref = None
+    for ps in (P_SMALL, 4 * 10**9):
+        device_sync()
+        t0 = time.perf_counter()
+        st = ctx.verify_range(4, limit, ps, form=gb.GB_FORM_PLAIN, seg_evens=PLAIN_SEG)
+        device_sync()
+        dt = time.perf_counter() - t0
+        assert st["evens"] == evens and st["failures"] == 0 and st["phase2"] == 0
+        ref = ref or st
+        assert all(st[k] == ref[k] for k in ("evens", "phase2", "failures"))
+        out[f"p_small_{ps:.0e}".replace("+", "")] = {"seconds": dt, "evens_per_s": evens / dt,
+                                                     "kernel_ms": st["kernel_ms"],
+                                                     "vs_reference": REF_GPU_SECONDS[limit] / dt}
+    return out
+
+
+def limit_1e9(ctx, gb, ref_cpu: bool) -> dict:
+    """All even n in [4, 1e9] in the plain and batched forms; with ref_cpu, also the reference's
+    own cpu_goldbach 1000000000 (one thread, built from its sources into oracle/_ref)."""
+    limit, evens = 10**9, (10**9 - 4) // 2 + 1
+    out = {"workload": "all even n in [4, 1e9]", "reference_gpu_seconds": REF_GPU_SECONDS[limit]}
+    for name, kw in (("plain", {"form": gb.GB_FORM_PLAIN, "seg_evens": PLAIN_SEG}),
+                     ("batched", {"form": gb.GB_FORM_BATCHED})):
+        ctx.verify_range(4, limit, P_SMALL, **kw)                                   # warm
+        device_sync()
+        t0 = time.perf_counter()
+        st = ctx.verify_range(4, limit, P_SMALL, **kw)
+        device_sync()
+        dt = time.perf_counter() - t0
+        assert st["evens"] == evens and st["failures"] == 0 and st["phase2"] == 0
+        out[name] = {"seconds": dt, "evens_per_s": evens / dt, "vs_reference": REF_GPU_SECONDS[limit] / dt}
+    if ref_cpu and os.path.exists(REF_CPU_GOLDBACH):
+        t0 = time.perf_counter()
+        r = subprocess.run([REF_CPU_GOLDBACH, str(limit)], capture_output=True, text=True, timeout=900)
+        dt = time.perf_counter() - t0
+        assert r.returncode == 0 and "satisfy Goldbach" in r.stdout, r.stdout + r.stderr
+        out["reference_cpu"] = {"seconds": dt, "evens_per_s": evens / dt, "threads": 1,
+                                "program": "oracle/_ref/cpu_goldbach 1000000000 (cpu_goldbach.cpp + "
+                                           "segmented_sieve.cpp, g++ -O3)"}
+    return out
 
 
 def presieve_density() -> float:
@@ -333,6 +391,8 @@ def main() -> int:
     ap.add_argument("--no-windows", action="store_true", help="skip the high-window (Baillie-PSW) leg")
     ap.add_argument("--no-counts", action="store_true", help="skip the partition-count leg")
     ap.add_argument("--no-big", action="store_true", help="skip the big_check (10^1000, 10^10000) leg")
+    ap.add_argument("--ref-cpu", action="store_true",
+                    help="also time the reference's own cpu_goldbach 1000000000 (one thread, about a minute)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -386,7 +446,8 @@ def main() -> int:
         psteps = max(1, min(args.steps, 2))
         pdt, pouts = timed_passes(plain_pass, psteps, 0)
         plain = {"value": evens * psteps / pdt, "ms_per_step": pdt * 1e3 / psteps,
-                 "kernel_ms": float(np.mean([o["kernel_ms"] for o in pouts])), "seg_evens": PLAIN_SEG}
+                 "kernel_ms": float(np.mean([o["kernel_ms"] for o in pouts])), "seg_evens": PLAIN_SEG,
+                 "limit_1e11": plain_1e11(ctx, gb), "limit_1e9": limit_1e9(ctx, gb, args.ref_cpu)}
 
     serve = None if args.no_serve else serving(ctx)
     high = None if args.no_windows else windows_leg(ctx)
