@@ -23,7 +23,9 @@ The JSON line carries:
                 reference algorithm: per-n ascending prime scan) timed on this
                 host over 64 sampled 2^24-even segments spread over [4, 1e12].
   high_windows  64 windows above 1e15 (q by pre-sieve + Baillie-PSW) in one
-                persistent launch, evens/s.
+                persistent launch, evens/s, with Montgomery products/s against
+                the VALU issue ceiling; its CPU baseline is the oracle's
+                Miller-Rabin restatement on the first 2^17 evens of each window.
   partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
                 candidate pairs/s (the reference loop's p = 2..n/2 per n), with
                 the oracle's c(n) on 512 n below 3e7 as its CPU baseline.
@@ -33,7 +35,8 @@ The JSON line carries:
   serving       the continuous-batching server (gb_serve_*): p50/p99 latency
                 from scheduled arrival to result under open-loop Poisson
                 arrivals at 1e4, 1e5 and 1e6 queries/s, and queries/s for a
-                2^20-query burst; n log-uniform in [1e6, 2^64-2].
+                2^20-query burst; n log-uniform in [1e6, 2^64-2]; its CPU
+                baseline is the oracle's min_partition over 2^19 of the queries.
 """
 from __future__ import annotations
 
@@ -107,6 +110,45 @@ def cpu_baseline() -> dict:
     dt = time.perf_counter() - t0
     return {"value": evens / dt, "unit": "evens/s", "cores": threads, "kind": "port",
             "sample": f"64 segments of 2^24 evens spread over [4, 1e12], {threads} threads, {dt:.1f} s wall"}
+
+
+def cpu_windows_baseline() -> dict:
+    """The oracle (u128 Miller-Rabin restatement) on the first 2^17 evens of each of the 64 high windows."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from _helpers import Oracle
+
+    orc = Oracle()
+    threads = min(16, os.cpu_count() or 1)
+    seg = 1 << 17
+    t0 = time.perf_counter()
+
+    def one(w):
+        st, _, _ = orc.verify_segment(w[0], seg, P_SMALL, want_bits=False, want_pmin=False)
+        assert st["failures"] == 0
+        return st["evens"]
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        evens = sum(ex.map(one, high_windows()))
+    dt = time.perf_counter() - t0
+    return {"value": evens / dt, "unit": "evens/s", "cores": threads, "kind": "port",
+            "sample": f"the first 2^17 evens of each of the 64 windows, {threads} threads, {dt:.1f} s wall"}
+
+
+def cpu_serving_baseline() -> dict:
+    """The oracle's min_partition over 2^19 of the serving leg's queries (OpenMP over queries)."""
+    from _helpers import Oracle
+
+    orc = Oracle()
+    threads = min(16, os.cpu_count() or 1)
+    ns = serving_ns(1 << 19, seed=1)
+    t0 = time.perf_counter()
+    p = orc.min_partition_batch(ns, threads=threads)
+    dt = time.perf_counter() - t0
+    assert int((p == 0).sum()) == 0
+    return {"value": ns.shape[0] / dt, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"2^19 queries of the serving stream (n log-uniform in [1e6, 2^64-2], seed 1), "
+                      f"{threads} threads, {dt:.1f} s wall"}
 
 
 def serving_ns(count: int, seed: int = 1) -> np.ndarray:
@@ -459,6 +501,10 @@ def main() -> int:
         counts["cpu_baseline"] = cpu_counts_baseline()
     if big is not None and not args.no_cpu:
         big["cpu_baseline"] = cpu_big_baseline()
+    if high is not None and not args.no_cpu:
+        high["cpu_baseline"] = cpu_windows_baseline()
+    if serve is not None and not args.no_cpu:
+        serve["cpu_baseline"] = cpu_serving_baseline()
 
     line = {
         "metric": METRIC,
