@@ -25,7 +25,7 @@ The JSON line carries:
   high_windows  64 windows above 1e15 (q by pre-sieve + Baillie-PSW) in one
                 persistent launch, evens/s, with Montgomery products/s against
                 the VALU issue ceiling; its CPU baseline is the oracle's
-                Miller-Rabin restatement on the first 2^17 evens of each window.
+                Miller-Rabin restatement on the first 2^19 evens of each window.
   partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
                 candidate pairs/s (the reference loop's p = 2..n/2 per n), with
                 the oracle's c(n) on 512 n below 3e7 as its CPU baseline.
@@ -36,7 +36,7 @@ The JSON line carries:
                 from scheduled arrival to result under open-loop Poisson
                 arrivals at 1e4, 1e5 and 1e6 queries/s, and queries/s for a
                 2^20-query burst; n log-uniform in [1e6, 2^64-2]; its CPU
-                baseline is the oracle's min_partition over 2^19 of the queries.
+                baseline is the oracle's min_partition over 2^21 queries of the stream.
 """
 from __future__ import annotations
 
@@ -113,14 +113,14 @@ def cpu_baseline() -> dict:
 
 
 def cpu_windows_baseline() -> dict:
-    """The oracle (u128 Miller-Rabin restatement) on the first 2^17 evens of each of the 64 high windows."""
+    """The oracle (u128 Miller-Rabin restatement) on the first 2^19 evens of each of the 64 high windows."""
     from concurrent.futures import ThreadPoolExecutor
 
     from _helpers import Oracle
 
     orc = Oracle()
     threads = min(16, os.cpu_count() or 1)
-    seg = 1 << 17
+    seg = 1 << 19
     t0 = time.perf_counter()
 
     def one(w):
@@ -132,22 +132,22 @@ def cpu_windows_baseline() -> dict:
         evens = sum(ex.map(one, high_windows()))
     dt = time.perf_counter() - t0
     return {"value": evens / dt, "unit": "evens/s", "cores": threads, "kind": "port",
-            "sample": f"the first 2^17 evens of each of the 64 windows, {threads} threads, {dt:.1f} s wall"}
+            "sample": f"the first 2^19 evens of each of the 64 windows, {threads} threads, {dt:.1f} s wall"}
 
 
 def cpu_serving_baseline() -> dict:
-    """The oracle's min_partition over 2^19 of the serving leg's queries (OpenMP over queries)."""
+    """The oracle's min_partition over 2^21 queries of the serving stream (OpenMP over queries)."""
     from _helpers import Oracle
 
     orc = Oracle()
     threads = min(16, os.cpu_count() or 1)
-    ns = serving_ns(1 << 19, seed=1)
+    ns = serving_ns(1 << 21, seed=1)
     t0 = time.perf_counter()
     p = orc.min_partition_batch(ns, threads=threads)
     dt = time.perf_counter() - t0
     assert int((p == 0).sum()) == 0
     return {"value": ns.shape[0] / dt, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"2^19 queries of the serving stream (n log-uniform in [1e6, 2^64-2], seed 1), "
+            "sample": f"2^21 queries of the serving stream (n log-uniform in [1e6, 2^64-2], seed 1), "
                       f"{threads} threads, {dt:.1f} s wall"}
 
 
