@@ -131,7 +131,9 @@ def test_full_size_1e11_plain_equals_batched(ctx, gb):
 
 def test_windows_match_oracle(ctx, orc):
     wins = [(10**15, 20_000), (10**18 + 2, 5_000), (1234567890123456, 7_777), (2**64 - 2 - 2 * 3999, 4000),
-            (4, 3000), (3325581707333960528 - 600, 700)]
+            (4, 3000), (3325581707333960528 - 600, 700),
+            (2**44 - 20_000, 20_000),                      # q and n on both sides of 2^44
+            (2**44 - 2**21, (1 << 20) + 5)]                # tiles below 2^44 (wheel) and above (Baillie-PSW)
     per, tot = ctx.verify_windows(wins, 10**6, want_pmin=True)
     evens = 0
     for (a, e), got in zip(wins, per):
