@@ -28,7 +28,7 @@
  * Parity pins (checked in tests/test_oracle.py):
  *   pi(10^k) k=1..8 and primes <= 30  -- src/test_sieve.cpp:31-40, src/test_bitset.cpp:17-50
  *   byte/word equality with the reference's own segmented_sieve / build_prime_bitset
- *     compiled from /root/reference into oracle/_ref (where that tree is present)
+ *     compiled from the reference tree into oracle/_ref (where that tree is present)
  *   cpu_goldbach binary from oracle/_ref on the same LIMIT
  *   minimal-p record table to 1e8 (OEIS A025018/A025019) and literature p_min values
  *   published partitions RESULTS.md:114-121 (both parts prime)

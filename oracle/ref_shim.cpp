@@ -1,7 +1,7 @@
 // ref_shim.cpp -- extern "C" entry points over the reference's own host
 // library functions, so tests can call them through ctypes.  Built only into
 // oracle/_ref/ (never shipped): the reference sources are compiled where they
-// lie under /root/reference by oracle/Makefile; nothing of them is copied.
+// lie in the reference tree by oracle/Makefile; nothing of them is copied.
 //
 // Wrapped (reference file:line):
 //   simple_sieve(uint64_t)                 src/segmented_sieve.cpp:13-26
