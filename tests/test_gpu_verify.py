@@ -200,3 +200,22 @@ def test_min_partition_matches_oracle(ctx, orc):
     # each result is the same alone and in the mix
     for i in (0, 17, 2999):
         assert ctx.min_partition(n[i:i + 1])[0] == got[i]
+
+
+def _range_fixtures():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "range_sums.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+def test_range_sums_equal_fixture(ctx, gb, form):
+    # [4, 1e9], [4, 1e10] and [1e10, 1e10 + 2^31] against the oracle's statistics; their
+    # largest p_min are published records (721013438: 1789, 7473202036: 1877, 11001080372: 1879)
+    f = gb.GB_FORM_BATCHED if form == "batched" else gb.GB_FORM_PLAIN
+    for r in _range_fixtures():
+        st = ctx.verify_range(r["start"], r["limit"], r["p_small"], form=f, want_pmin=True)
+        for k in ("evens", "phase2", "failures", "first_fail_n", "sum_pmin", "max_pmin", "argmax_n"):
+            assert st[k] == r[k], (r["start"], r["limit"], k)
+

@@ -1,0 +1,42 @@
+"""Range statistics fixtures (tests/golden/range_sums.json, written by the oracle through
+tests/golden/make_range_sums.py) and the published minimal-partition records they contain.
+
+The largest p_min of each range is a record of T. Oliveira e Silva's table of minimal
+Goldbach partitions (OEIS A025018 for n, A025019 for p): 1789 at 721,013,438; 1877 at
+7,473,202,036; 1879 at 11,001,080,372.  The oracle's per-n scan (cpu_goldbach.cpp:19-22
+order) must give exactly those, and [4, 1e9] must be reproducible from the oracle here."""
+import json
+import os
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+RECORDS = [(721_013_438, 1789), (7_473_202_036, 1877), (11_001_080_372, 1879)]
+
+
+def fixtures():
+    with open(os.path.join(HERE, "golden", "range_sums.json")) as f:
+        return json.load(f)
+
+
+def test_fixture_shape():
+    for r in fixtures():
+        assert r["evens"] == (r["limit"] - r["start"]) // 2 + 1
+        assert r["phase2"] == 0 and r["failures"] == 0 and r["first_fail_n"] == 0
+        assert r["start"] <= r["argmax_n"] <= r["limit"]
+
+
+def test_records_are_the_ranges_maxima(orc):
+    got = {(r["argmax_n"], r["max_pmin"]) for r in fixtures()}
+    assert got == set(RECORDS)
+    for n, p in RECORDS:
+        assert orc.min_partition(n) == p
+        assert orc.is_prime(p) and orc.is_prime(n - p)
+
+
+@pytest.mark.parametrize("idx", [0])
+def test_fixture_reproduces(orc, idx):
+    r = fixtures()[idx]
+    st = orc.verify_range(r["start"], r["limit"], 1 << 27, r["p_small"])
+    for k in ("evens", "phase2", "failures", "first_fail_n", "sum_pmin", "max_pmin", "argmax_n"):
+        assert st[k] == r[k], k
