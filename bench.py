@@ -434,7 +434,7 @@ def main() -> int:
     ap.add_argument("--no-counts", action="store_true", help="skip the partition-count leg")
     ap.add_argument("--no-big", action="store_true", help="skip the big_check (10^1000, 10^10000) leg")
     ap.add_argument("--ref-cpu", action="store_true",
-                    help="also time the reference's own cpu_goldbach 1000000000 (one thread, about a minute)")
+                    help="also time the reference's own cpu_goldbach 1000000000 (one thread, 10-60 s)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
     args = ap.parse_args()
