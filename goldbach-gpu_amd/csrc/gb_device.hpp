@@ -384,6 +384,23 @@ __host__ __device__ constexpr uint64_t jacobi_mask(uint32_t m, int v) {
     return s;
 }
 
+// x^{-1} mod c for x = 1..c-1 (c <= 15) as 4-bit fields at 4x, 0 where no inverse exists
+__host__ __device__ constexpr uint64_t inv_nibbles(uint32_t c) {
+    uint64_t s = 0;
+    for (uint32_t x = 1; x < c; ++x)
+        for (uint32_t y = 1; y < c; ++y)
+            if (x * y % c == 1) s |= (uint64_t)y << (4 * x);
+    return s;
+}
+
+// c^{-1} mod 2^64 for odd c (Newton: each step doubles the correct low bits)
+__host__ __device__ constexpr uint64_t inv_pow2_64(uint64_t c) {
+    uint64_t x = c;
+    for (int i = 0; i < 6; ++i) x *= 2 - c * x;
+    return x;
+}
+static_assert(inv_nibbles(7) == 0x6325410ull && inv_pow2_64(3) * 3 == 1 && inv_pow2_64(11) * 11 == 1, "inverse tables");
+
 // Selfridge's candidates D = 5, -7, 9, ..., -47 (|D| <= 47).  Each is 1 mod 4, so by
 // reciprocity (D/n) = (n mod |D| / |D|) for every odd n; n mod |D| comes from n modulo one
 // of 218243025 = 3^3 5^2 7 11 13 17 19, 31367009 = 23 29 31 37 41 and 2021 = 43 47.
@@ -440,6 +457,7 @@ __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
             if ((SELFRIDGE[k].minus >> a) & 1) D = SELFRIDGE[k].D;
         }
     }
+    const bool D_from_table = D != 0;
     if (D == 0) {
         const uint64_t r = isqrt64(n);
         if (r * r == n) return 0;
@@ -461,7 +479,18 @@ __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
     const uint32_t e = __builtin_ctz(aq), c = aq >> e;
     uint64_t qi = M.one;
     for (uint32_t i = 0; i < e; ++i) qi = half_mod(qi, M.n);
-    if (c > 1) {
+    if (c > 1 && D_from_table) {
+        // |D| <= 47: c is 3, 5, 7, 9 or 11, all dividing 218243025; the inverses mod c and
+        // mod 2^64 come from compile-time constants, selected without a branch
+        const uint64_t tab = c == 3 ? inv_nibbles(3) : c == 5 ? inv_nibbles(5) : c == 7 ? inv_nibbles(7)
+                           : c == 9 ? inv_nibbles(9) : inv_nibbles(11);
+        const uint64_t ci = c == 3 ? inv_pow2_64(3) : c == 5 ? inv_pow2_64(5) : c == 7 ? inv_pow2_64(7)
+                          : c == 9 ? inv_pow2_64(9) : inv_pow2_64(11);
+        const uint32_t inv = (uint32_t)(tab >> (4 * (res[0] % c))) & 15u;
+        if (!inv) return 0;                                                     // Q shares a factor with n
+        const uint32_t t = (c - (uint32_t)(qi % 218243025ull) % c * inv % c) % c;
+        qi = (qi + (uint64_t)t * M.n) * ci;
+    } else if (c > 1) {
         const uint32_t c32 = (0xFFFFFFFFu % c + 1) % c;
         const uint32_t inv = inv_mod32(mod_u64_small(n, c, c32), c);
         if (!inv) return 0;                                                     // Q shares a factor with n

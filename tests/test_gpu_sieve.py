@@ -40,14 +40,17 @@ def test_strong_pseudoprime_is_cleared(ctx, orc, q):
     assert np.array_equal(ctx.sieve_window(q - 20_000, q + 20_000), orc.window_bits(q - 20_000, q + 20_000))
 
 
-# Primes 1 + k * 47#/2 (k = 28, 38): 1 modulo every odd prime up to 47, so (D/q) = 1 for
-# every Selfridge candidate with |D| <= 47 and the strong Lucas test has to search on
-# (to D = 53 and D = -59).
-LATE_D = [8608456956238879741, 11682905869181336791]
+# Primes above 2^44 by the Selfridge parameter D their strong Lucas test takes, one for each
+# odd part c of |Q| = |1 - D|/4 > 1 in the device's D table (built by the Chinese remainder
+# theorem: q = 1 modulo every odd prime below |D|, a non-residue modulo |D|): D = 29 (c = 7),
+# 37 (c = 9), 41 (c = 5), -43 (c = 11), -47 (c = 3); and primes 1 + k * 47#/2 (k = 28, 38),
+# 1 modulo every odd prime up to 47, whose search runs past the table (D = 53 and -59).
+LATE_D = [1125910450544881, 1133367329724631, 5276144813849911, 53852296644316171, 3480014514224227981,
+          8608456956238879741, 11682905869181336791]
 
 
 @pytest.mark.parametrize("q", LATE_D)
-def test_lucas_parameter_beyond_47(ctx, orc, q):
+def test_lucas_parameter_cases(ctx, orc, q):
     assert orc.is_prime(q)
     bits = ctx.sieve_window(q - 2 * 64, q + 2 * 64)
     assert unpack(bits, 129)[64] == 1
