@@ -11,6 +11,9 @@
 //      their new position.
 // Submitters and pollers only touch host queues under a mutex, so arrivals
 // overlap with the step running on the device and join at the next step.
+// An idle engine polls an arrival counter for up to 200 us before it sleeps on
+// its condition variable, so a query arriving at an idle server does not wait for
+// the thread to be woken (the price: one host core busy while queries keep coming).
 //
 // Replaces single_check's flow (single_check.cu:136-319 of the reference:
 // CPU sieve of p chunks, one launch per chunk, first finder wins) with a
@@ -19,6 +22,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -35,6 +39,7 @@ namespace {
 
 constexpr uint32_t DEFAULT_CHUNKS = 4;
 constexpr uint64_t DEFAULT_QUEUE = 1ull << 22;
+constexpr uint64_t IDLE_SPIN_NS = 200000;   // an idle engine polls this long for arrivals before it sleeps
 constexpr uint64_t SLOT_BYTES = sizeof(ServeIn) + sizeof(ServeOut);   // device state per slot
 
 uint64_t now_ns() {
@@ -72,6 +77,8 @@ struct gb_server {
     std::deque<Query> queue;
     std::deque<gb_result> done;
     uint64_t next_id = 0;
+    std::atomic<uint64_t> arrivals{0};   // submissions so far: read without the lock by an idle engine
+    std::atomic<bool> stopping{false};
     bool stop = false;
     bool failed = false;
     std::string err;
@@ -95,10 +102,20 @@ void engine_main(gb_server* s) {
     }
     std::vector<Query> active;
     active.reserve(s->slots);
+    uint64_t seen = 0, left = 0;   // submissions counted and queries left queued at the last admission
     for (;;) {
+        if (active.empty() && left == 0) {
+            // idle: poll for arrivals for a short while, so a query that arrives now does not
+            // wait for the thread to be woken; then sleep on the condition variable
+            const uint64_t t0 = now_ns();
+            while (s->arrivals.load(std::memory_order_acquire) == seen && !s->stopping.load(std::memory_order_acquire) &&
+                   now_ns() - t0 < IDLE_SPIN_NS) {
+            }
+        }
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !active.empty(); });
+            seen = s->arrivals.load(std::memory_order_relaxed);
             if (s->stop) return;
             while (active.size() < s->slots && !s->queue.empty()) {
                 active.push_back(s->queue.front());
@@ -106,6 +123,7 @@ void engine_main(gb_server* s) {
             }
             s->st.queued = s->queue.size();
             s->st.resident = active.size();
+            left = s->queue.size();
         }
         const uint32_t m = (uint32_t)active.size();
         for (uint32_t i = 0; i < m; ++i) s->h_in[i] = ServeIn{active[i].n, active[i].k};
@@ -229,6 +247,7 @@ int gb_serve_submit(gb_server* s, const uint64_t* n, uint32_t count, uint64_t* f
     }
     if (first_id) *first_id = s->next_id;
     for (uint32_t i = 0; i < count; ++i) s->queue.push_back(Query{s->next_id++, n[i], 0, t, 0});
+    s->arrivals.fetch_add(count, std::memory_order_release);
     s->st.submitted += count;
     s->st.queued = s->queue.size();
     if (count) s->cv_work.notify_one();
@@ -264,6 +283,7 @@ int gb_serve_close(gb_server* s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
+        s->stopping.store(true, std::memory_order_release);
     }
     s->cv_work.notify_all();
     if (s->engine.joinable()) s->engine.join();
