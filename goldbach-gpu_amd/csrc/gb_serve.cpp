@@ -1,9 +1,11 @@
 // gb_serve.cpp -- the continuous-batching serving engine of goldbach_gpu.h.
 //
-// One engine thread per server owns a HIP stream and a table of device slots.
+// One engine thread per server owns a HIP stream and a slot table in pinned host
+// memory that the step kernel reads and writes directly (a few bytes per query,
+// so no copy commands sit between a launch and its answers).
 // Each iteration (a "step") it
 //   1. moves queued queries into free slots (at most `slots` resident: the
-//      slot count is what the device-memory budget allows),
+//      slot count is what the memory budget allows),
 //   2. launches serve_step_kernel over every resident query, new and
 //      in-flight alike (one wave per query, up to chunks_per_step chunks of
 //      64 candidate primes from the query's saved position),
@@ -40,7 +42,7 @@ namespace {
 constexpr uint32_t DEFAULT_CHUNKS = 4;
 constexpr uint64_t DEFAULT_QUEUE = 1ull << 22;
 constexpr uint64_t IDLE_SPIN_NS = 200000;   // an idle engine polls this long for arrivals before it sleeps
-constexpr uint64_t SLOT_BYTES = sizeof(ServeIn) + sizeof(ServeOut);   // device state per slot
+constexpr uint64_t SLOT_BYTES = sizeof(ServeIn) + sizeof(ServeOut);   // slot state (pinned, device-visible)
 
 uint64_t now_ns() {
     timespec ts;
@@ -67,9 +69,7 @@ struct gb_server {
     uint32_t chunks = 0;
     uint64_t queue_cap = 0;
     hipStream_t stream = nullptr;
-    ServeIn* d_in = nullptr;
-    ServeOut* d_out = nullptr;
-    ServeIn* h_in = nullptr;    // pinned staging
+    ServeIn* h_in = nullptr;    // slot state in pinned host memory, read and written by the step kernel
     ServeOut* h_out = nullptr;
     std::mutex mu;
     std::condition_variable cv_work;   // engine waits for queries or stop
@@ -127,10 +127,9 @@ void engine_main(gb_server* s) {
         }
         const uint32_t m = (uint32_t)active.size();
         for (uint32_t i = 0; i < m; ++i) s->h_in[i] = ServeIn{active[i].n, active[i].k};
-        hipError_t e = hipMemcpyAsync(s->d_in, s->h_in, m * sizeof(ServeIn), hipMemcpyHostToDevice, s->stream);
-        if (e == hipSuccess) e = launch_serve_step(s->stream, s->T, s->d_in, m, s->bound, s->chunks, s->d_out);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(s->h_out, s->d_out, m * sizeof(ServeOut), hipMemcpyDeviceToHost, s->stream);
+        // the step reads its inputs from and writes its results to the pinned staging directly:
+        // a few bytes per query, so no copy commands sit between the launch and the answer
+        hipError_t e = launch_serve_step(s->stream, s->T, s->h_in, m, s->bound, s->chunks, s->h_out);
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) {
             engine_fail(s, std::string("serving engine: ") + hipGetErrorString(e));
@@ -169,8 +168,6 @@ void engine_main(gb_server* s) {
 
 void free_server(gb_server* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    if (s->d_in) (void)hipFree(s->d_in);
-    if (s->d_out) (void)hipFree(s->d_out);
     if (s->h_in) (void)hipHostFree(s->h_in);
     if (s->h_out) (void)hipHostFree(s->h_out);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -207,8 +204,6 @@ int gb_serve_open(gb_ctx* c, const gb_serve_config* cfg, gb_server** out) {
     s->queue_cap = k.queue_cap ? k.queue_cap : DEFAULT_QUEUE;
     s->st.slots = s->slots;
     hipError_t e = hipStreamCreate(&s->stream);
-    if (e == hipSuccess) e = dalloc(&s->d_in, slots);
-    if (e == hipSuccess) e = dalloc(&s->d_out, slots);
     if (e == hipSuccess)
         e = hipHostMalloc(reinterpret_cast<void**>(&s->h_in), slots * sizeof(ServeIn), hipHostMallocDefault);
     if (e == hipSuccess)

@@ -271,7 +271,8 @@ typedef struct gb_server gb_server;
 typedef struct gb_serve_config {
     uint32_t slots;            /* resident queries per step (0 = the context's serve_slots) */
     uint32_t chunks_per_step;  /* 64-prime chunks per query per step (0 = 4) */
-    uint64_t mem_budget;       /* device bytes for slot state; slots shrink to fit (0 = no limit) */
+    uint64_t mem_budget;       /* bytes of slot state (40 per slot, pinned host memory the step
+                                  kernel reads and writes); slots shrink to fit (0 = no limit) */
     uint64_t queue_cap;        /* queries waiting for a slot before submit says GB_EAGAIN (0 = 2^22) */
 } gb_serve_config;
 
