@@ -166,6 +166,14 @@ __device__ __forceinline__ uint32_t first_off(double y, uint32_t p, double inv) 
     return u;
 }
 
+// a[i] through a 32-bit byte offset from the table's (uniform) base, so the load can take
+// the scalar-base + 32-bit vector-offset form instead of a 64-bit address per lane (every
+// table read this way holds well under 2^32 bytes)
+template <class T>
+__device__ __forceinline__ T ld32(const T* __restrict__ a, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a) + (i * (uint32_t)sizeof(T)));
+}
+
 template <int BS>
 __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ scratch, uint32_t scratch_cap,
                            uint32_t* __restrict__ wsum, uint64_t qv0, uint32_t lead, uint64_t g0, uint32_t nbits,
@@ -258,14 +266,14 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     {
         uint32_t u = tid, it_n = 0, p_n = 0;
         if (u < n_items) {
-            it_n = T.items[u];
-            p_n = T.items_p[u];
+            it_n = ld32(T.items, u);
+            p_n = ld32(T.items_p, u);
         }
         while (u < n_items) {
             const uint32_t it = it_n, p = p_n, un = u + BS;
             if (un < n_items) {
-                it_n = T.items[un];
-                p_n = T.items_p[un];
+                it_n = ld32(T.items, un);
+                p_n = ld32(T.items_p, un);
             }
             const uint32_t f = med_i0[(it >> 12) - NPAT];
             const uint32_t i = (f & 0x7FFFFFFFu) + (it & 4095u) * (CHUNK_SPAN * p);
@@ -287,11 +295,11 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
         const uint64_t root_lo = isqrt64(q_lo);
         uint32_t k = T.n_med + tid;
         PrimeRes nx{0, 0, 0.0};
-        if (k < np) nx = pr[k];
+        if (k < np) nx = ld32(pr, k);
         while (k < np) {
             const PrimeRes e = nx;
             const uint32_t kn = k + BS;
-            if (kn < np) nx = pr[kn];
+            if (kn < np) nx = ld32(pr, kn);
             uint32_t i0 = lb + first_off((double)e.e - dbf, e.p, e.inv);
             if (e.p > root_lo) {
                 const uint64_t pp = (uint64_t)e.p * e.p;
@@ -886,14 +894,14 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
     {
         uint32_t u = n_wave + tid, it_n = 0, p_n = 0;
         if (u < n_items) {
-            it_n = T.items[u];
-            p_n = T.items_p[u];
+            it_n = ld32(T.items, u);
+            p_n = ld32(T.items_p, u);
         }
         while (u < n_items) {
             const uint32_t it = it_n, p = p_n, un = u + BS;
             if (un < n_items) {
-                it_n = T.items[un];
-                p_n = T.items_p[un];
+                it_n = ld32(T.items, un);
+                p_n = ld32(T.items_p, un);
             }
             const uint32_t sidx = (it >> 12) & 1u;
             uint32_t* S = wh + sidx * SW;
@@ -912,15 +920,15 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
         PrimeRes nx{0, 0, 0.0};
         uint32_t d_n = 0;
         if (k < np) {
-            nx = pr[k];
-            d_n = T.d5[k];
+            nx = ld32(pr, k);
+            d_n = ld32(T.d5, k);
         }
         while (k < np) {
             const PrimeRes e = nx;
             const uint32_t d5 = d_n, kn = k + BS;
             if (kn < np) {
-                nx = pr[kn];
-                d_n = T.d5[kn];
+                nx = ld32(pr, kn);
+                d_n = ld32(T.d5, kn);
             }
             uint32_t o1 = first_off((double)e.e - dbf, e.p, e.inv);
             uint32_t o5 = o1 + d5;
