@@ -174,7 +174,10 @@ __device__ __forceinline__ T ld32(const T* __restrict__ a, uint32_t i) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a) + (i * (uint32_t)sizeof(T)));
 }
 
-template <int BS>
+// MR = false leaves the Baillie-PSW step out of the instantiation, for windows whose q all lie
+// below (cover + 1)^2 (the callers check): the strong tests would otherwise set the register
+// budget, and so the occupancy, of every plain sieve launch.
+template <int BS, bool MR = true>
 __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ scratch, uint32_t scratch_cap,
                            uint32_t* __restrict__ wsum, uint64_t qv0, uint32_t lead, uint64_t g0, uint32_t nbits,
                            const Tables& T, uint32_t tid, const PrimeRes* __restrict__ pr, uint64_t rbase) {
@@ -266,14 +269,14 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     {
         uint32_t u = tid, it_n = 0, p_n = 0;
         if (u < n_items) {
-            it_n = ld32(T.items, u);
-            p_n = ld32(T.items_p, u);
+            it_n = T.items[u];
+            p_n = T.items_p[u];
         }
         while (u < n_items) {
             const uint32_t it = it_n, p = p_n, un = u + BS;
             if (un < n_items) {
-                it_n = ld32(T.items, un);
-                p_n = ld32(T.items_p, un);
+                it_n = T.items[un];
+                p_n = T.items_p[un];
             }
             const uint32_t f = med_i0[(it >> 12) - NPAT];
             const uint32_t i = (f & 0x7FFFFFFFu) + (it & 4095u) * (CHUNK_SPAN * p);
@@ -295,11 +298,11 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
         const uint64_t root_lo = isqrt64(q_lo);
         uint32_t k = T.n_med + tid;
         PrimeRes nx{0, 0, 0.0};
-        if (k < np) nx = ld32(pr, k);
+        if (k < np) nx = pr[k];
         while (k < np) {
             const PrimeRes e = nx;
             const uint32_t kn = k + BS;
-            if (kn < np) nx = ld32(pr, kn);
+            if (kn < np) nx = pr[kn];
             uint32_t i0 = lb + first_off((double)e.e - dbf, e.p, e.inv);
             if (e.p > root_lo) {
                 const uint64_t pp = (uint64_t)e.p * e.p;
@@ -316,8 +319,8 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
 
     // 4) above 2^44 the table cannot reach sqrt(q): survivors of the pre-sieve
     //    are decided by Baillie-PSW (exact below 2^64).
-    if (mr && any_valid) {
-        mr_tile<BS>(win32, nbits, qv0 + 2 * g0 - 2ull * lead, tid, scratch, scratch_cap, wsum);
+    if constexpr (MR) {
+        if (mr && any_valid) mr_tile<BS>(win32, nbits, qv0 + 2 * g0 - 2ull * lead, tid, scratch, scratch_cap, wsum);
     }
 }
 
@@ -673,7 +676,7 @@ __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first,
 // ---------------------------------------------------------------------------
 // plain sieve kernel: one tile of tile_bits window bits per workgroup
 // ---------------------------------------------------------------------------
-template <int BS>
+template <int BS, bool MR>
 __global__ __launch_bounds__(BS) void sieve_window_kernel(Tables T, uint64_t qv0, uint32_t lead, uint64_t nbits,
                                                           uint32_t tile_bits, uint64_t* __restrict__ out,
                                                           int tail_ones, const PrimeRes* __restrict__ pr) {
@@ -685,7 +688,7 @@ __global__ __launch_bounds__(BS) void sieve_window_kernel(Tables T, uint64_t qv0
     const uint64_t g0 = (uint64_t)blockIdx.x * tile_bits;
     if (g0 >= nbits) return;
     const uint32_t tb = (uint32_t)(nbits - g0 < tile_bits ? nbits - g0 : tile_bits);
-    sieve_tile<BS>(win32, scratch, SIEVE_MRQ, wsum, qv0, lead, g0, tb, T, tid, pr, qv0);
+    sieve_tile<BS, MR>(win32, scratch, SIEVE_MRQ, wsum, qv0, lead, g0, tb, T, tid, pr, qv0);
     const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
     const uint32_t nw = (tb + 63) >> 6;
     const uint64_t wo = g0 >> 6;
@@ -1512,10 +1515,18 @@ hipError_t launch_sieve_window(hipStream_t s, const Tables& T, uint64_t qv0, uin
     if (ntiles > 0x7FFFFFFFull || tile_bits > SIEVE_TILE_MAX || (tile_bits & 63)) return hipErrorInvalidValue;
     const uint32_t med = T.n_med > NPAT ? T.n_med - NPAT : 1;
     const size_t lds = (size_t)(SIEVE_BS / 64 + (med > SIEVE_MRQ ? med : SIEVE_MRQ)) * 4;
-    hipError_t e = allow_lds(sieve_window_kernel<SIEVE_BS>, lds);
+    // the largest q of the window: bit nbits - 1 (bits below `lead` stand for no q)
+    const uint64_t span = nbits - 1 >= lead ? nbits - 1 - lead : 0;
+    const uint64_t c1 = (uint64_t)T.cover + 1;
+    const bool mr = span > (~0ull - qv0) / 2 || qv0 + 2 * span >= c1 * c1;
+    hipError_t e = mr ? allow_lds(sieve_window_kernel<SIEVE_BS, true>, lds) : allow_lds(sieve_window_kernel<SIEVE_BS, false>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(sieve_window_kernel<SIEVE_BS>, dim3((uint32_t)ntiles), dim3(SIEVE_BS), lds, s, T, qv0, lead,
-                       nbits, tile_bits, out, tail_ones, pr);
+    if (mr)
+        hipLaunchKernelGGL((sieve_window_kernel<SIEVE_BS, true>), dim3((uint32_t)ntiles), dim3(SIEVE_BS), lds, s, T, qv0,
+                           lead, nbits, tile_bits, out, tail_ones, pr);
+    else
+        hipLaunchKernelGGL((sieve_window_kernel<SIEVE_BS, false>), dim3((uint32_t)ntiles), dim3(SIEVE_BS), lds, s, T, qv0,
+                           lead, nbits, tile_bits, out, tail_ones, pr);
     return hipGetLastError();
 }
 
