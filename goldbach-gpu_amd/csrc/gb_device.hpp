@@ -172,7 +172,15 @@ struct Mont {
 // half of ab is formed once: m = lo(ab) * n^{-1} takes three word products and the high half
 // of mn four, since its low half equals lo(ab) (left to itself the compiler rebuilds lo(ab)
 // inside m = a * (b * n^{-1})).
-__device__ __forceinline__ uint64_t mont_redc(uint64_t hi, uint32_t t0, uint32_t t1, const Mont& M) {
+//
+// LAZY (only for n < 2^62): operands and results lie in [0, 2n) instead of [0, n).  Then
+// ab < 4n^2 < Rn, so (ab - mn)/R still lies in (-n, n), and adding n unconditionally leaves
+// it in (0, 2n): the compare and the two selects of the exact form drop out of every product.
+//
+// With an offset c in [n, 3n] and operands in [0, 4n) (n < 2^60, so ab < 16n^2 < Rn),
+// (ab - mn)/R + c lies in (0, 4n): c = 3n - x subtracts x < n inside the product.
+template <bool LAZY = false>
+__device__ __forceinline__ uint64_t mont_redc(uint64_t hi, uint32_t t0, uint32_t t1, const Mont& M, uint64_t off) {
     const uint32_t n0 = (uint32_t)M.n, n1 = (uint32_t)(M.n >> 32);
     const uint32_t i0 = (uint32_t)M.ninv, i1 = (uint32_t)(M.ninv >> 32);
     const uint64_t q = (uint64_t)t0 * i0;
@@ -180,28 +188,62 @@ __device__ __forceinline__ uint64_t mont_redc(uint64_t hi, uint32_t t0, uint32_t
     const uint64_t u = (uint64_t)m1 * n0 + __umulhi(m0, n0);     // the low word of m0 n0 is t0
     const uint64_t v = (uint64_t)m0 * n1 + (uint32_t)u;
     const uint64_t mh = (uint64_t)m1 * n1 + (u >> 32) + (v >> 32);
+    if constexpr (LAZY) return hi - mh + off;
     const uint64_t t = hi - mh;
     return hi < mh ? t + M.n : t;
 }
 
+template <bool LAZY = false>
+__device__ __forceinline__ uint64_t mont_redc(uint64_t hi, uint32_t t0, uint32_t t1, const Mont& M) {
+    return mont_redc<LAZY>(hi, t0, t1, M, M.n);
+}
+
+template <bool LAZY = false>
+__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, const Mont& M, uint64_t off) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0;
+    const uint64_t x = (uint64_t)a1 * b0 + (p00 >> 32);
+    const uint64_t y = (uint64_t)a0 * b1 + (uint32_t)x;
+    const uint64_t hi = (uint64_t)a1 * b1 + (x >> 32) + (y >> 32);
+    return mont_redc<LAZY>(hi, (uint32_t)p00, (uint32_t)y, M, off);
+}
+
+template <bool LAZY = false>
+__device__ __forceinline__ uint64_t mont_sqr(uint64_t a, const Mont& M, uint64_t off) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint64_t p00 = (uint64_t)a0 * a0;
+    const uint64_t c = (uint64_t)a0 * a1;
+    const uint64_t y = (uint64_t)(uint32_t)(c << 1) + (p00 >> 32);
+    const uint64_t hi = (uint64_t)a1 * a1 + (c >> 31) + (y >> 32);
+    return mont_redc<LAZY>(hi, (uint32_t)p00, (uint32_t)y, M, off);
+}
+
+template <bool LAZY = false>
 __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, const Mont& M) {
     const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
     const uint64_t p00 = (uint64_t)a0 * b0;
     const uint64_t x = (uint64_t)a1 * b0 + (p00 >> 32);
     const uint64_t y = (uint64_t)a0 * b1 + (uint32_t)x;
     const uint64_t hi = (uint64_t)a1 * b1 + (x >> 32) + (y >> 32);
-    return mont_redc(hi, (uint32_t)p00, (uint32_t)y, M);
+    return mont_redc<LAZY>(hi, (uint32_t)p00, (uint32_t)y, M);
 }
 
 // a^2: the cross product a0 a1 once, doubled
+template <bool LAZY = false>
 __device__ __forceinline__ uint64_t mont_sqr(uint64_t a, const Mont& M) {
     const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
     const uint64_t p00 = (uint64_t)a0 * a0;
     const uint64_t c = (uint64_t)a0 * a1;
     const uint64_t y = (uint64_t)(uint32_t)(c << 1) + (p00 >> 32);
     const uint64_t hi = (uint64_t)a1 * a1 + (c >> 31) + (y >> 32);
-    return mont_redc(hi, (uint32_t)p00, (uint32_t)y, M);
+    return mont_redc<LAZY>(hi, (uint32_t)p00, (uint32_t)y, M);
 }
+
+constexpr uint64_t MONT_LAZY_MAX = 1ull << 62;    // [0, 2n) needs 4n <= R
+constexpr uint64_t MONT_LAZY4_MAX = 1ull << 60;   // [0, 4n) needs 16n <= R
+
+// x in [0, 2n) to [0, n)
+__device__ __forceinline__ uint64_t mont_canon(uint64_t x, uint64_t n) { return x >= n ? x - n : x; }
 
 __device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t n) {
     // a, b < n: (a + b) mod n without 65-bit overflow
@@ -255,14 +297,33 @@ __device__ __forceinline__ bool sprp(uint64_t a, uint64_t d, uint32_t r, const M
     return false;
 }
 
-// Strong test to base 2: the multiply step of the ladder is a doubling.
-__device__ __forceinline__ bool sprp2(uint64_t d, uint32_t r, const Mont& M) {
-    uint64_t x = add_mod(M.one, M.one, M.n);   // 2 in Montgomery form
+// 2^d in Montgomery form by a left-to-right ladder whose multiply step is a doubling, the
+// result in [0, n).  MODE 0: exact products.  MODE 1 (n < 2^62): the ladder runs in
+// [0, 2n).  MODE 2 (n < 2^60): it runs in [0, 4n), where a product lands in (0, 2n) and
+// its doubling needs no reduction.
+template <int MODE>
+__device__ __forceinline__ uint64_t pow2_ladder(uint64_t d, const Mont& M) {
+    constexpr bool LAZY = MODE != 0;
+    const uint64_t md = LAZY ? 2 * M.n : M.n;   // MODE 0, 1: the ladder's values lie in [0, md)
+    uint64_t x = add_mod(M.one, M.one, M.n);    // 2 in Montgomery form
     const int top = 63 - __clzll((long long)d);
     for (int b = top - 1; b >= 0; --b) {
-        x = mont_sqr(x, M);
-        if ((d >> b) & 1) x = add_mod(x, x, M.n);
+        x = mont_sqr<LAZY>(x, M);
+        if constexpr (MODE == 2) {
+            if ((d >> b) & 1) x += x;
+        } else {
+            if ((d >> b) & 1) x = add_mod(x, x, md);
+        }
     }
+    if constexpr (MODE == 2) x = mont_canon(x, 2 * M.n);
+    return LAZY ? mont_canon(x, M.n) : x;
+}
+
+// Strong test to base 2.
+__device__ __forceinline__ bool sprp2(uint64_t d, uint32_t r, const Mont& M) {
+    uint64_t x = M.n < MONT_LAZY4_MAX  ? pow2_ladder<2>(d, M)
+                 : M.n < MONT_LAZY_MAX ? pow2_ladder<1>(d, M)
+                                       : pow2_ladder<0>(d, M);
     const uint64_t minus_one = M.n - M.one;
     if (x == M.one || x == minus_one) return true;
     for (uint32_t i = 1; i < r; ++i) {
@@ -446,6 +507,43 @@ static_assert(218243025u == 27u * 25u * 7u * 11u * 13u * 17u * 19u && 31367009u 
 // A step of a Q = 1 sequence, (W_k, W_{k+1}) -> (W_{2k}, W_{2k+1}) or (W_{2k+1}, W_{2k+2}),
 // is one product and one square whichever the bit (W_{2k} = W_k^2 - 2, W_{2k+1} = W_k W_{k+1}
 // - P'), so lanes whose bits differ do not diverge.
+// (W_k, W_{k+1}) -> (W_{2k}, W_{2k+1}) or (W_{2k+1}, W_{2k+2}) over bits b..0 of m, the
+// values in [0, n) on entry and return.  MODE 0: exact products.  MODE 1 (n < 2^62): the
+// ladder runs in [0, 2n).  MODE 2 (n < 2^60): it runs in [0, 4n) and the subtractions of
+// P' and 2 ride in the products' offsets.
+template <int MODE>
+__device__ __forceinline__ void w_ladder(uint64_t m, int b, uint64_t& w0, uint64_t& w1, uint64_t pp, uint64_t two,
+                                         const Mont& M) {
+    if constexpr (MODE == 2) {
+        const uint64_t c1 = 3 * M.n - pp, c2 = 3 * M.n - two;
+        for (; b >= 0; --b) {
+            const bool bit = (m >> b) & 1;
+            const uint64_t x = mont_mul<true>(w0, w1, M, c1);                 // W_{2k+1}
+            const uint64_t h = bit ? w1 : w0;
+            const uint64_t y = mont_sqr<true>(h, M, c2);                      // W_{2k+2} or W_{2k}
+            w0 = bit ? x : y;
+            w1 = bit ? y : x;
+        }
+        w0 = mont_canon(mont_canon(w0, 2 * M.n), M.n);
+        w1 = mont_canon(mont_canon(w1, 2 * M.n), M.n);
+    } else {
+        constexpr bool LAZY = MODE == 1;
+        const uint64_t md = LAZY ? 2 * M.n : M.n;   // the ladder's values lie in [0, md)
+        for (; b >= 0; --b) {
+            const bool bit = (m >> b) & 1;
+            const uint64_t x = sub_mod(mont_mul<LAZY>(w0, w1, M), pp, md);    // W_{2k+1}
+            const uint64_t h = bit ? w1 : w0;
+            const uint64_t y = sub_mod(mont_sqr<LAZY>(h, M), two, md);       // W_{2k+2} or W_{2k}
+            w0 = bit ? x : y;
+            w1 = bit ? y : x;
+        }
+        if (LAZY) {
+            w0 = mont_canon(w0, M.n);
+            w1 = mont_canon(w1, M.n);
+        }
+    }
+}
+
 __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
     const uint32_t res[3] = {(uint32_t)(n % 218243025ull), (uint32_t)(n % 31367009ull), (uint32_t)(n % 2021ull)};
     int64_t D = 0;                                                              // 0: undecided
@@ -512,14 +610,12 @@ __device__ inline int strong_lucas(uint64_t n, const Mont& M) {
         w1 = sub_mod(mont_sqr(pp, M), two, M.n);
         b = 62 - __clzll((long long)m);
     }
-    for (; b >= 0; --b) {
-        const bool bit = (m >> b) & 1;
-        const uint64_t x = sub_mod(mont_mul(w0, w1, M), pp, M.n);              // W_{2k+1}
-        const uint64_t h = bit ? w1 : w0;
-        const uint64_t y = sub_mod(mont_sqr(h, M), two, M.n);               // W_{2k+2} or W_{2k}
-        w0 = bit ? x : y;
-        w1 = bit ? y : x;
-    }
+    if (M.n < MONT_LAZY4_MAX)
+        w_ladder<2>(m, b, w0, w1, pp, two, M);
+    else if (M.n < MONT_LAZY_MAX)
+        w_ladder<1>(m, b, w0, w1, pp, two, M);
+    else
+        w_ladder<0>(m, b, w0, w1, pp, two, M);
     if (w0 == w1 || add_mod(w0, w1, M.n) == 0) return 1;                       // U_d = 0 or V_d = 0
     uint64_t w = sub_mod(mont_mul(w0, w1, M), pp, M.n);                         // W_d
     for (uint32_t i = 1; i < s; ++i) {
