@@ -61,10 +61,14 @@ __device__ __forceinline__ uint32_t clear_mask(uint32_t i) { return __builtin_am
 __device__ __forceinline__ void lds_clear_bit(uint32_t* win32, uint32_t i) { atomicAnd(&win32[i >> 5], clear_mask(i)); }
 
 // bits i, i + p, ... below end
+// rot = 0 - p serves as the mask's rotation (left by p mod 32) and, negated, as the step; it
+// is formed in asm so that the compiler neither folds i - rot back into i + p nor recomputes
+// rot inside the loop when registers run short (which would cost a VALU op per mark).
 __device__ __forceinline__ void clear_run(uint32_t* win32, uint32_t i, uint32_t p, uint32_t end) {
     uint32_t mask = clear_mask(i);
-    const uint32_t rot = 0u - p;   // rotate left by p mod 32
-    for (; i < end; i += p) {
+    uint32_t rot;
+    asm("v_sub_u32 %0, 0, %1" : "=v"(rot) : "v"(p));
+    for (; i < end; i -= rot) {
         atomicAnd(&win32[i >> 5], mask);
         mask = __builtin_amdgcn_alignbit(mask, mask, rot);
     }
@@ -907,12 +911,13 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
                 p_n = ld32(T.items_p, un);
             }
             const uint32_t sidx = (it >> 12) & 1u;
-            uint32_t* S = wh + sidx * SW;
+            const uint32_t sb = sidx * (32 * SW);   // the stream's first bit in wh (a multiple of 32)
             const uint32_t i = med0[2 * ((it >> 13) - NPAT) + sidx] + (it & 4095u) * (CHUNK * p);
+            // marks indexed from wh itself: no per-mark add of the stream's base address
             if (i + (CHUNK - 1) * p < nbits)
-                clear_run_n<CHUNK>(S, i, p);
+                clear_run_n<CHUNK>(wh, sb + i, p);
             else
-                clear_run(S, i, p, nbits);
+                clear_run(wh, sb + i, p, sb + nbits);
             u = un;
         }
     }
