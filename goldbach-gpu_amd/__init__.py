@@ -37,8 +37,9 @@ EXPORTS = (
     "gb_min_partition_batch", "gb_autotune", "gb_count_partitions", "gb_verify_range_logged", "gb_log_read",
     "gb_big_min_partition", "gb_big_from_decimal", "gb_big_to_decimal",
     "gb_serve_open", "gb_serve_submit", "gb_serve_poll", "gb_serve_stats_get", "gb_serve_close",
-    "gb_serve_last_error",
+    "gb_serve_last_error", "gb_range_plan",
 )
+GB_KERNEL_PLAIN, GB_KERNEL_WHEEL, GB_KERNEL_FUSED = 0, 1, 2
 
 u64 = ctypes.c_uint64
 u32 = ctypes.c_uint32
@@ -59,6 +60,14 @@ class GbSegStats(ctypes.Structure):
 
 class GbTuning(ctypes.Structure):
     _fields_ = [("form", u32), ("tile_log2", u32), ("seg_evens", u64), ("wg_per_cu", u32), ("flags", u32)]
+
+
+class GbChunk(ctypes.Structure):
+    _fields_ = [("first_n", u64), ("last_n", u64), ("tiles", u64), ("kernel", u32), ("tile_log2", u32),
+                ("wg_per_cu", u32), ("reserved", u32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 class GbWindow(ctypes.Structure):
@@ -158,6 +167,8 @@ def library() -> ctypes.CDLL:
                                        u64, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
         "gb_autotune": (ctypes.c_int, [vp, u64, u64, u64, u32, u32, ctypes.POINTER(GbTuning),
                                        ctypes.POINTER(ctypes.c_double)]),
+        "gb_range_plan": (ctypes.c_int, [vp, u64, u64, u64, ctypes.POINTER(GbTuning), ctypes.POINTER(GbChunk), u64,
+                                         ctypes.POINTER(u64)]),
         "gb_serve_open": (ctypes.c_int, [vp, ctypes.POINTER(GbServeConfig), ctypes.POINTER(vp)]),
         "gb_serve_submit": (ctypes.c_int, [vp, vp, u32, ctypes.POINTER(u64)]),
         "gb_serve_poll": (ctypes.c_int, [vp, vp, u32, ctypes.POINTER(u32), u32]),
@@ -294,13 +305,26 @@ class Context:
     # -- whole range -------------------------------------------------------
     def verify_range(self, start: int, limit: int, p_small: int, form: int = GB_FORM_BATCHED,
                      tile_log2: int = 0, seg_evens: int = 0, wg_per_cu: int = 0, want_pmin: bool = False,
-                     autotune: bool = False) -> dict:
+                     autotune: bool = False, progress: Optional[ctypes.c_uint64] = None) -> dict:
+        """progress (optional): a ctypes.c_uint64 the library updates with the evens finished."""
         flags = (GB_WANT_PMIN if want_pmin else 0) | (GB_TUNE_AUTO if autotune else 0)
         tun = GbTuning(form, tile_log2, seg_evens, wg_per_cu, flags)
         st = GbSegStats()
         self._check(self._lib.gb_verify_range(self._h, start, limit, p_small, ctypes.byref(tun), ctypes.byref(st),
-                                              None))
+                                              ctypes.byref(progress) if progress is not None else None))
         return st.as_dict()
+
+    def range_plan(self, start: int, limit: int, p_small: int = 1_000_000, form: int = GB_FORM_BATCHED,
+                   tile_log2: int = 0, seg_evens: int = 0, wg_per_cu: int = 0, autotune: bool = False) -> list:
+        """gb_range_plan: the segments or launches verify_range would run, as dicts."""
+        tun = GbTuning(form, tile_log2, seg_evens, wg_per_cu, GB_TUNE_AUTO if autotune else 0)
+        n = u64(0)
+        self._check(self._lib.gb_range_plan(self._h, start, limit, p_small, ctypes.byref(tun), None, 0,
+                                            ctypes.byref(n)))
+        arr = (GbChunk * max(n.value, 1))()
+        self._check(self._lib.gb_range_plan(self._h, start, limit, p_small, ctypes.byref(tun), arr, n.value,
+                                            ctypes.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
 
     def verify_range_logged(self, start: int, limit: int, p_small: int, log_path: str, chunk_evens: int = 0,
                             max_chunks: int = 0, form: int = GB_FORM_BATCHED, tile_log2: int = 0,

@@ -39,10 +39,6 @@ constexpr uint64_t COUNT_N_MAX = 1ull << 36;             // partition counts: bi
 std::mutex g_open_mu;
 std::string g_open_err;
 
-void dfree(void* p) {
-    if (p) (void)hipFree(p);
-}
-
 void dhostfree(void* p) {
     if (p) (void)hipHostFree(p);
 }
@@ -58,6 +54,33 @@ namespace gbh {
 int fail(gb_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     return code;
+}
+
+int hip_fail(gb_ctx* c, hipError_t e, const char* file, int line) {
+    if (e == hipErrorOutOfMemory) {
+        char buf[256];
+        if (c && c->budget_hit) {
+            c->budget_hit = false;
+            std::snprintf(buf, sizeof buf, "device memory budget exceeded (%llu MB held of %llu MB)",
+                          (unsigned long long)(c->mem_used >> 20), (unsigned long long)(c->budget >> 20));
+        } else {
+            std::snprintf(buf, sizeof buf, "device out of memory (%llu MB held by the context)",
+                          (unsigned long long)(c ? c->mem_used >> 20 : 0));
+        }
+        return fail(c, GB_ENOMEM, buf);
+    }
+    return fail(c, GB_EDEVICE, std::string("HIP error at ") + file + ":" + std::to_string(line) + " -- " +
+                                   hipGetErrorString(e));
+}
+
+void dfree(gb_ctx* c, void* p) {
+    if (!p) return;
+    (void)hipFree(p);
+    auto it = c->held.find(p);
+    if (it != c->held.end()) {
+        c->mem_used -= it->second;
+        c->held.erase(it);
+    }
 }
 
 // Medium-prime schedule for tiles of at most `tb` bits: prime k with
@@ -98,9 +121,9 @@ int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel) {
     t.n_med = k;
     // start[k'] for k' in [0, n_med]: items of primes with index < k'
     start.resize(t.n_med + 1);
-    HCK(c, dalloc(&t.d_items, items.size()));
-    HCK(c, dalloc(&t.d_items_p, items_p.size()));
-    HCK(c, dalloc(&t.d_start, start.size()));
+    HCK(c, dalloc(c, &t.d_items, items.size()));
+    HCK(c, dalloc(c, &t.d_items_p, items_p.size()));
+    HCK(c, dalloc(c, &t.d_start, start.size()));
     if (!items.empty()) {
         HCK(c, hipMemcpy(t.d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice));
         HCK(c, hipMemcpy(t.d_items_p, items_p.data(), items_p.size() * 4, hipMemcpyHostToDevice));
@@ -192,10 +215,10 @@ void add_stats(gb_seg_stats* a, const gb_seg_stats& b) {
 
 int ensure_r0(gb_ctx* c, uint64_t n) {
     if (n <= c->r0_cap) return GB_OK;
-    dfree(c->d_r0);
+    dfree(c, c->d_r0);
     c->d_r0 = nullptr;
     c->r0_cap = 0;
-    HCK(c, dalloc(&c->d_r0, n));
+    HCK(c, dalloc(c, &c->d_r0, n));
     c->r0_cap = n;
     return GB_OK;
 }
@@ -218,24 +241,31 @@ void set_residues(const gb_ctx* c, WinDesc* wd, uint64_t rbase, uint64_t q_top, 
 
 int ensure_wins(gb_ctx* c, uint32_t n) {
     if (n <= c->wins_cap) return GB_OK;
-    dfree(c->d_wins);
-    dfree(c->d_rec_lo);
-    dfree(c->d_rec_hi);
-    dfree(c->d_out);
-    HCK(c, dalloc(&c->d_wins, n));
-    HCK(c, dalloc(&c->d_rec_lo, n));
-    HCK(c, dalloc(&c->d_rec_hi, n));
-    HCK(c, dalloc(&c->d_out, n));
+    dfree(c, c->d_wins);
+    dfree(c, c->d_rec_lo);
+    dfree(c, c->d_rec_hi);
+    dfree(c, c->d_out);
+    HCK(c, dalloc(c, &c->d_wins, n));
+    HCK(c, dalloc(c, &c->d_rec_lo, n));
+    HCK(c, dalloc(c, &c->d_rec_hi, n));
+    HCK(c, dalloc(c, &c->d_out, n));
     if (c->h_out) dhostfree(c->h_out);
+    if (c->h_wins) dhostfree(c->h_wins);
+    if (c->h_lohi) dhostfree(c->h_lohi);
+    c->h_out = nullptr;
+    c->h_wins = nullptr;
+    c->h_lohi = nullptr;
     HCK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_out), n * sizeof(StatRec), hipHostMallocDefault));
+    HCK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_wins), n * sizeof(WinDesc), hipHostMallocDefault));
+    HCK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_lohi), 2ull * n * sizeof(uint64_t), hipHostMallocDefault));
     c->wins_cap = n;
     return GB_OK;
 }
 
 int ensure_rec(gb_ctx* c, uint64_t n) {
     if (n <= c->rec_cap) return GB_OK;
-    dfree(c->d_rec);
-    HCK(c, dalloc(&c->d_rec, n));
+    dfree(c, c->d_rec);
+    HCK(c, dalloc(c, &c->d_rec, n));
     c->rec_cap = n;
     return GB_OK;
 }
@@ -251,11 +281,11 @@ int build_tables(gb_ctx* c) {
     uint32_t* d_seed_c32 = nullptr;
     uint32_t* d_cnt = nullptr;
     const uint32_t seed_words = ((SEED_LIMIT - 3) / 2 + 1 + 63) / 64;
-    HCK(c, dalloc(&d_seedbits, seed_words));
-    HCK(c, dalloc(&d_seed, 1024));
-    HCK(c, dalloc(&d_seed_inv, 1024));
-    HCK(c, dalloc(&d_seed_c32, 1024));
-    HCK(c, dalloc(&d_cnt, 2));
+    HCK(c, dalloc(c, &d_seedbits, seed_words));
+    HCK(c, dalloc(c, &d_seed, 1024));
+    HCK(c, dalloc(c, &d_seed_inv, 1024));
+    HCK(c, dalloc(c, &d_seed_c32, 1024));
+    HCK(c, dalloc(c, &d_cnt, 2));
     HCK(c, launch_seed_primes(s, SEED_LIMIT, d_seedbits));
     HCK(c, launch_compact_primes(s, d_seedbits, seed_words, SEED_LIMIT, d_seed, 1024, d_cnt));
     uint32_t nseed = 0;
@@ -282,34 +312,34 @@ int build_tables(gb_ctx* c) {
     const uint64_t sp_bits = (SP_MAX - 1 - 3) / 2 + 1;   // odd q in [3, 2^22 - 1]
     const uint64_t sp_words = (sp_bits + 63) / 64;
     uint64_t* d_spbits = nullptr;
-    HCK(c, dalloc(&d_spbits, sp_words));
+    HCK(c, dalloc(c, &d_spbits, sp_words));
     // the seed primes as the window [3, 2^22) sees them
     WinDesc* d_w = nullptr;
     PrimeRes* d_pr = nullptr;
-    HCK(c, dalloc(&d_w, 1));
-    HCK(c, dalloc(&d_pr, nseed));
+    HCK(c, dalloc(c, &d_w, 1));
+    HCK(c, dalloc(c, &d_pr, nseed));
     const WinDesc w_seed{3, 0, 0, 0, 3, 0, nseed, 0, 0};
     HCK(c, hipMemcpyAsync(d_w, &w_seed, sizeof w_seed, hipMemcpyHostToDevice, s));
     HCK(c, launch_residues(s, seedT, d_w, 1, nseed, d_pr));
     HCK(c, launch_sieve_window(s, seedT, 3, 0, sp_bits, PLAIN_TILE_BITS, d_spbits, 0, d_pr));
-    HCK(c, dalloc(&c->d_sp, SP_TABLE_CAP));
+    HCK(c, dalloc(c, &c->d_sp, SP_TABLE_CAP));
     HCK(c, launch_compact_primes(s, d_spbits, (uint32_t)sp_words, SP_MAX - 1, c->d_sp, SP_TABLE_CAP, d_cnt));
     HCK(c, hipMemcpyAsync(&c->nsp, d_cnt, 4, hipMemcpyDeviceToHost, s));
     HCK(c, hipStreamSynchronize(s));
     if (c->nsp == 0 || c->nsp > SP_TABLE_CAP) return fail(c, GB_EDEVICE, "prime table build produced a bad count");
-    HCK(c, dalloc(&c->d_inv, c->nsp));
-    HCK(c, dalloc(&c->d_c32, c->nsp));
-    HCK(c, dalloc(&c->d_d5, c->nsp));
+    HCK(c, dalloc(c, &c->d_inv, c->nsp));
+    HCK(c, dalloc(c, &c->d_c32, c->nsp));
+    HCK(c, dalloc(c, &c->d_d5, c->nsp));
     HCK(c, launch_prime_aux(s, c->d_sp, c->nsp, c->d_inv, c->d_c32, c->d_d5));
     c->h_sp.resize(c->nsp);
     HCK(c, hipMemcpyAsync(c->h_sp.data(), c->d_sp, c->nsp * 4ull, hipMemcpyDeviceToHost, s));
     HCK(c, hipStreamSynchronize(s));
-    dfree(d_seedbits);
-    dfree(d_seed);
-    dfree(d_seed_inv);
-    dfree(d_seed_c32);
-    dfree(d_cnt);
-    dfree(d_spbits);
+    dfree(c, d_seedbits);
+    dfree(c, d_seed);
+    dfree(c, d_seed_inv);
+    dfree(c, d_seed_c32);
+    dfree(c, d_cnt);
+    dfree(c, d_spbits);
 
     // Phase-2 bitset: odd primes <= phase2_limit
     ItemTable it;
@@ -320,22 +350,22 @@ int build_tables(gb_ctx* c) {
     if (!(lim & 1)) lim -= 1;
     c->p2_limit = lim;
     const uint64_t p2_bits = (lim - 3) / 2 + 1;
-    HCK(c, dalloc(&c->d_p2bits, (p2_bits + 63) / 64));
+    HCK(c, dalloc(c, &c->d_p2bits, (p2_bits + 63) / 64));
     Tables T = make_tables(c, it);
     T.p2bits = nullptr;
     T.p2_limit = 0;
     WinDesc w_p2{3, 0, 0, 0, 0, 0, 0, 0, 0};
     set_residues(c, &w_p2, 3, lim, 0);
     HCK(c, hipStreamSynchronize(s));
-    dfree(d_pr);
+    dfree(c, d_pr);
     d_pr = nullptr;
-    HCK(c, dalloc(&d_pr, w_p2.np));
+    HCK(c, dalloc(c, &d_pr, w_p2.np));
     HCK(c, hipMemcpyAsync(d_w, &w_p2, sizeof w_p2, hipMemcpyHostToDevice, s));
     HCK(c, launch_residues(s, T, d_w, 1, w_p2.np, d_pr));
     HCK(c, launch_sieve_window(s, T, 3, 0, p2_bits, PLAIN_TILE_BITS, c->d_p2bits, 0, d_pr));
     HCK(c, hipStreamSynchronize(s));
-    dfree(d_w);
-    dfree(d_pr);
+    dfree(c, d_w);
+    dfree(c, d_pr);
     return GB_OK;
 }
 
@@ -352,10 +382,10 @@ int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool 
     const uint64_t vwords = (cnt + 63) / 64;
     if (wwords > c->window_cap || vwords > c->verified_cap) return fail(c, GB_ERANGE, "segment larger than max_seg_evens");
     if (want_pmin && cnt > c->pmin_cap) {
-        dfree(c->d_pmin);
+        dfree(c, c->d_pmin);
         c->d_pmin = nullptr;
         c->pmin_cap = 0;
-        HCK(c, dalloc(&c->d_pmin, cnt));
+        HCK(c, dalloc(c, &c->d_pmin, cnt));
         c->pmin_cap = cnt;
     }
     const uint64_t nblocks = (vwords + MARK_WPB - 1) / MARK_WPB;
@@ -463,7 +493,9 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
     const uint64_t p2_bytes = (c->cfg.phase2_limit / 2 / 64 + 2) * 8;
     const uint64_t fixed = 300000ull * 16 + p2_bytes + SLOW_CAP * 28ull + (64ull << 20);
     const uint64_t required = (window_words + verified_words) * 8 + fixed + VRAM_MARGIN;
-    const uint64_t budget = c->cfg.mem_budget ? std::min<uint64_t>(c->cfg.mem_budget, c->total_mem) : c->total_mem;
+    const uint64_t budget = c->cfg.mem_budget ? std::min<uint64_t>(c->cfg.mem_budget, c->total_mem)
+                                              : c->total_mem - std::min<uint64_t>(c->total_mem / 2, 1ull << 30);
+    c->budget = budget;
     if (required > budget) {
         char buf[256];
         std::snprintf(buf, sizeof buf, "insufficient VRAM: required %llu MB, available %llu MB",
@@ -478,10 +510,10 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
     }
     int rc = build_tables(c);
     if (rc) return bail(rc);
-    if (dalloc(&c->d_window, window_words) != hipSuccess || dalloc(&c->d_verified, verified_words) != hipSuccess ||
-        dalloc(&c->d_slow_n, SLOW_CAP) != hipSuccess || dalloc(&c->d_slow_win, SLOW_CAP) != hipSuccess ||
-        dalloc(&c->d_slow_res, SLOW_CAP) != hipSuccess || dalloc(&c->d_slow_ctl, 2) != hipSuccess ||
-        dalloc(&c->d_counter, 1) != hipSuccess) {
+    if (dalloc(c, &c->d_window, window_words) != hipSuccess || dalloc(c, &c->d_verified, verified_words) != hipSuccess ||
+        dalloc(c, &c->d_slow_n, SLOW_CAP) != hipSuccess || dalloc(c, &c->d_slow_win, SLOW_CAP) != hipSuccess ||
+        dalloc(c, &c->d_slow_res, SLOW_CAP) != hipSuccess || dalloc(c, &c->d_slow_ctl, 2) != hipSuccess ||
+        dalloc(c, &c->d_counter, 1) != hipSuccess) {
         c->err = "device allocation failed";
         return bail(GB_ENOMEM);
     }
@@ -502,18 +534,21 @@ int gb_close(gb_ctx* c) {
     if (c->servers.load() > 0) return fail(c, GB_EAGAIN, "gb_close: close the context's servers first");
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (auto& g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (auto& it : c->items) {
-        dfree(it.d_items);
-        dfree(it.d_items_p);
-        dfree(it.d_start);
+        dfree(c, it.d_items);
+        dfree(c, it.d_items_p);
+        dfree(c, it.d_start);
     }
     void* bufs[] = {c->d_sp, c->d_inv, c->d_c32, c->d_d5, c->d_p2bits, c->d_window, c->d_verified, c->d_pmin, c->d_rec,
                     c->d_out, c->d_slow_n, c->d_slow_win, c->d_slow_ctl, c->d_slow_res, c->d_wins, c->d_rec_lo,
                     c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0};
     for (void* b : bufs)
-        if (b) dfree(b);
+        if (b) dfree(c, b);
     if (c->h_out) dhostfree(c->h_out);
+    if (c->h_wins) dhostfree(c->h_wins);
+    if (c->h_lohi) dhostfree(c->h_lohi);
     if (c->h_ctl) dhostfree(c->h_ctl);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -576,144 +611,280 @@ int gb_verify_segment(gb_ctx* c, uint64_t A, uint64_t cnt, uint64_t p_small, uin
     return GB_OK;
 }
 
-// Batched persistent form over windows; records land in c->h_out[0..nwin).
-static int run_batched(gb_ctx* c, const std::vector<WinDesc>& wins_in, uint64_t p_small, uint32_t flags,
-                       uint32_t tile_log2, uint32_t wg_per_cu, uint64_t* verified_dev, double* ms) {
-    hipStream_t s = c->stream;
-    bool below = !verified_dev;   // every window where the wheel applies
-    for (const WinDesc& w : wins_in)
-        if (w.start + 2 * (w.evens - 1) > WHEEL_N_MAX) below = false;
-    if (tile_log2 == 0 && wg_per_cu == 0 && below) {
-        tile_log2 = WHEEL_TILE_LOG2;   // measured at [4, 1e12]: 181 ms against 198 ms for one 2^20 tile per CU
-        wg_per_cu = 2;
-    }
-    if (tile_log2 == 0) tile_log2 = DEFAULT_TILE_LOG2;
-    if (tile_log2 < 12 || tile_log2 > 20) return fail(c, GB_EINVAL, "tile_log2 must be in [12, 20]");
-    const uint32_t te = 1u << tile_log2;
-    const MarkParams mp = mark_params(c, p_small, flags);
-    const uint32_t nwin = (uint32_t)wins_in.size();
-    // The mod-6 wheel serves every window below 2^44 (where the table sieves q
-    // exactly) unless a verified map is wanted; its tiles hold 3 * 2^(tile_log2-1)
-    // even n.  Above, tiles pre-sieve and run Miller-Rabin on the odd-only window.
-    const bool wheel = below;
-    const uint32_t cls_len = te / 2;
-    if (wheel && wg_per_cu >= 2 && cls_len > WH_CLASS_HALF)
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched form.  A range (or a list of windows) is cut into pieces on the tile
+// grid of its kernel kind: the mod-6 wheel where every n is at most
+// WHEEL_N_MAX, the odd-only fused kernel above.  Pieces run in launches of at
+// most CHUNK_TILES tiles, so the per-tile records, prime entries and window
+// descriptors a launch needs stay bounded whatever the range's length (the
+// reference's fixed per-segment buffers, goldbach.cu:378-417).  A launch is a
+// graph whose kernels read their pieces from d_wins, so every chunk of the
+// same shape replays one graph; the host syncs once per launch, which is where
+// progress moves and a failure stops the range (goldbach.cu:420,476-480).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t CHUNK_TILES = 1ull << 17;   // tiles per launch at most: 8 MiB of records
+constexpr uint32_t LAUNCH_PIECES = 64;          // pieces per launch at most
+constexpr size_t GRAPH_CACHE = 8;
+
+struct Shape {
+    bool wheel;
+    uint32_t tile_log2;   // fused: tiles of 2^tile_log2 evens; wheel: 3 * 2^(tile_log2 - 1) evens
+    uint32_t wg;          // persistent workgroups per CU
+};
+
+// Even n [start, start + 2(evens - 1)] that belong to output `owner`.
+struct Piece {
+    uint64_t start, evens;
+    uint32_t owner;
+};
+
+// Tile shape of a kind: the defaults when neither knob is given, else the knobs (one workgroup per
+// CU and 2^20-even tiles where only one of them is).
+Shape shape_of(bool wheel, uint32_t tile_log2, uint32_t wg) {
+    if (tile_log2 == 0 && wg == 0)
+        return wheel ? Shape{true, WHEEL_TILE_LOG2, 2} : Shape{false, DEFAULT_TILE_LOG2, 1};
+    return Shape{wheel, tile_log2 ? tile_log2 : DEFAULT_TILE_LOG2, wg ? wg : 1};
+}
+
+int check_shape(gb_ctx* c, const Shape& sh) {
+    if (sh.tile_log2 < 12 || sh.tile_log2 > 20) return fail(c, GB_EINVAL, "tile_log2 must be in [12, 20]");
+    if (sh.wg > 16) return fail(c, GB_EINVAL, "wg_per_cu must be at most 16");
+    if (sh.wheel && sh.wg >= 2 && (1u << (sh.tile_log2 - 1)) > WH_CLASS_HALF)
         return fail(c, GB_EINVAL, "two workgroups per CU take tiles of at most 2^19 evens");
+    return GB_OK;
+}
+
+uint64_t tile_span(const Shape& sh) {   // n covered by one tile
+    return sh.wheel ? (6ull << (sh.tile_log2 - 1)) : (2ull << sh.tile_log2);
+}
+
+uint64_t piece_tiles(const Shape& sh, uint64_t start, uint64_t evens) {
+    const uint64_t n_hi = start + 2 * (evens - 1);
+    if (sh.wheel) return (n_hi - (start - start % 6)) / tile_span(sh) + 1;
+    const uint64_t te = 1ull << sh.tile_log2;
+    return (evens + te - 1) / te;
+}
+
+// [a, b] (even, a <= b) cut into pieces of at most cap tiles on the tile grid of sh that starts at
+// a (a wheel tile begins at a multiple of 6, so every later piece starts on its own tile).
+void cut_pieces(const Shape& sh, uint64_t a, uint64_t b, uint32_t owner, uint64_t cap, std::vector<Piece>& out) {
+    const uint64_t nt = piece_tiles(sh, a, (b - a) / 2 + 1);
+    const uint64_t base = sh.wheel ? a - a % 6 : a, span = tile_span(sh);
+    for (uint64_t t0 = 0; t0 < nt; t0 += cap) {
+        const uint64_t t1 = std::min(nt, t0 + cap);
+        const uint64_t lo = std::max(a, base + span * t0);
+        const uint64_t hi = t1 == nt ? b : base + span * t1 - 2;
+        out.push_back(Piece{lo, (hi - lo) / 2 + 1, owner});
+    }
+}
+
+// [a, b] split where the wheel stops serving (every n of a wheel part is at most WHEEL_N_MAX).
+void split_kinds(uint64_t a, uint64_t b, std::vector<std::pair<bool, std::pair<uint64_t, uint64_t>>>& out) {
+    if (a <= WHEEL_N_MAX) out.push_back({true, {a, std::min(b, WHEEL_N_MAX)}});
+    if (b > WHEEL_N_MAX) out.push_back({false, {std::max(a, WHEEL_N_MAX + 2), b}});
+}
+
+void drop_graphs(gb_ctx* c) {
+    for (auto& g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
+// One launch over pieces pcs[0 .. np) of shape sh; the per-piece records land in c->h_out[0 .. np).
+// *overflow: more n than the slow list holds had p_min beyond the window's reach (the launch's
+// results are then incomplete and it must run again in smaller pieces).
+int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64_t p_small, uint32_t flags,
+               double* ms, bool* overflow) {
+    hipStream_t s = c->stream;
+    const MarkParams mp = mark_params(c, p_small, flags);
+    const uint32_t cls_len = 1u << (sh.tile_log2 - 1), te = 1u << sh.tile_log2;
     ItemTable it;
-    int rc = wheel ? build_items(c, cls_len + WH_HALO + 64, &it, true) : build_items(c, te + HALO + WIN_PAD, &it);
+    int rc = sh.wheel ? build_items(c, cls_len + WH_HALO + 64, &it, true) : build_items(c, te + HALO + WIN_PAD, &it);
     if (rc) return rc;
     const Tables T = make_tables(c, it);
-    std::vector<WinDesc> wins(wins_in);
-    std::vector<uint64_t> lo(nwin), hi(nwin);
-    uint64_t tiles = 0, words = 0, nres = 0;
-    uint32_t max_np = 0;
-    for (uint32_t w = 0; w < nwin && wheel; ++w) {
-        const uint64_t start = wins[w].start, n_hi = start + 2 * (wins[w].evens - 1);
-        const uint64_t nb = start - start % 6;
-        const uint64_t nt = (n_hi - nb) / (6ull * cls_len) + 1;
-        wins[w].tile0 = tiles;
-        wins[w].word0 = 0;
-        lo[w] = tiles;
+    rc = ensure_wins(c, np);
+    if (rc) return rc;
+    uint64_t tiles = 0, nres = 0;
+    for (uint32_t w = 0; w < np; ++w) {
+        WinDesc& wd = c->h_wins[w];
+        wd = WinDesc{pcs[w].start, pcs[w].evens, 0, 0, 0, 0, 0, 0, 0};
+        const uint64_t nt = piece_tiles(sh, wd.start, wd.evens);
+        wd.tile0 = tiles;
+        c->h_lohi[w] = tiles;
         tiles += nt;
-        hi[w] = tiles;
-        wins[w].kbase = (int64_t)(nb / 6) - (int64_t)WH_HALO;
-        const int64_t k_last = wins[w].kbase + (int64_t)(cls_len * (nt - 1));
-        const uint64_t q_top = 6ull * (uint64_t)(k_last + cls_len + WH_HALO + 64 - 1) + 5;
-        wins[w].rbase = 0;
-        wins[w].r0off = nres;
-        wins[w].np = count_le(c->h_sp, sieve_prime_bound(q_top));
-        wins[w].pad = 0;
-        nres += wins[w].np;
-        max_np = std::max(max_np, wins[w].np);
+        c->h_lohi[np + w] = tiles;
+        if (sh.wheel) {
+            const uint64_t nb = wd.start - wd.start % 6;
+            wd.kbase = (int64_t)(nb / 6) - (int64_t)WH_HALO;
+            const int64_t k_last = wd.kbase + (int64_t)((uint64_t)cls_len * (nt - 1));
+            const uint64_t q_top = 6ull * (uint64_t)(k_last + cls_len + WH_HALO + 64 - 1) + 5;
+            wd.r0off = nres;
+            wd.np = count_le(c->h_sp, sieve_prime_bound(q_top));
+        } else {
+            // every tile's first sieved q is >= rbase; the last tile reaches q_top
+            const uint64_t rbase = wd.start >= 2ull * HALO + 4 ? wd.start - 2ull * HALO - 1 : 3;
+            const uint64_t last_n0 = wd.start + 2ull * te * ((wd.evens - 1) / te);
+            const uint64_t last_cnt = wd.evens - (uint64_t)te * ((wd.evens - 1) / te);
+            const WinGeom lg = window_geom(last_n0, last_cnt + HALO + WIN_PAD);
+            set_residues(c, &wd, rbase, geom_q_top(lg), nres);
+        }
+        nres += wd.np;
     }
-    for (uint32_t w = 0; w < nwin && !wheel; ++w) {
-        wins[w].tile0 = tiles;
-        wins[w].word0 = words;
-        lo[w] = tiles;
-        tiles += (wins[w].evens + te - 1) / te;
-        hi[w] = tiles;
-        words += (wins[w].evens + 63) / 64;
-        // every tile's first sieved q is >= rbase; the last tile reaches q_top
-        const uint64_t start = wins[w].start;
-        const uint64_t rbase = start >= 2ull * HALO + 4 ? start - 2ull * HALO - 1 : 3;
-        const uint64_t last_n0 = start + 2ull * te * ((wins[w].evens - 1) / te);
-        const uint64_t last_cnt = wins[w].evens - (uint64_t)te * ((wins[w].evens - 1) / te);
-        const WinGeom lg = window_geom(last_n0, last_cnt + HALO + WIN_PAD);
-        set_residues(c, &wins[w], rbase, geom_q_top(lg), nres);
-        nres += wins[w].np;
-        max_np = std::max(max_np, wins[w].np);
-    }
+    const void* old_bufs[] = {c->d_rec, c->d_r0};
     rc = ensure_rec(c, tiles);
     if (rc) return rc;
-    rc = ensure_r0(c, nres);
+    rc = ensure_r0(c, std::max<uint64_t>(nres, c->nsp));
     if (rc) return rc;
-    rc = ensure_wins(c, nwin);
-    if (rc) return rc;
-    HCK(c, hipMemcpyAsync(c->d_wins, wins.data(), nwin * sizeof(WinDesc), hipMemcpyHostToDevice, s));
-    HCK(c, hipMemcpyAsync(c->d_rec_lo, lo.data(), nwin * 8, hipMemcpyHostToDevice, s));
-    HCK(c, hipMemcpyAsync(c->d_rec_hi, hi.data(), nwin * 8, hipMemcpyHostToDevice, s));
-    if (wg_per_cu == 0) wg_per_cu = 1;
-    uint64_t grid = (uint64_t)c->cus * wg_per_cu;
+    if (old_bufs[0] != c->d_rec || old_bufs[1] != c->d_r0) drop_graphs(c);
+    HCK(c, hipMemcpyAsync(c->d_wins, c->h_wins, np * sizeof(WinDesc), hipMemcpyHostToDevice, s));
+    HCK(c, hipMemcpyAsync(c->d_rec_lo, c->h_lohi, np * 8ull, hipMemcpyHostToDevice, s));
+    HCK(c, hipMemcpyAsync(c->d_rec_hi, c->h_lohi + np, np * 8ull, hipMemcpyHostToDevice, s));
+    uint64_t grid = (uint64_t)c->cus * sh.wg;
     if (grid > tiles) grid = tiles;
     SlowList sl = slow_list(c);
-
-    std::vector<uint64_t> key;
-    for (auto& w : wins) { key.push_back(w.start); key.push_back(w.evens); }
-    key.push_back(p_small);
-    key.push_back(flags);
-    key.push_back(tile_log2);
-    key.push_back(wheel ? 1 : 0);
-    key.push_back(grid);
-    key.push_back((uint64_t)(uintptr_t)verified_dev);
-    key.push_back((uint64_t)(uintptr_t)c->d_rec);
-    key.push_back((uint64_t)(uintptr_t)c->d_wins);
-    key.push_back((uint64_t)(uintptr_t)c->d_out);
-    key.push_back((uint64_t)(uintptr_t)c->d_r0);
-    if (!c->graph || key != c->graph_key) {
-        if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
-        HCK(c, hipStreamSynchronize(s));
+    const std::vector<uint64_t> key = {sh.wheel ? 1ull : 0ull, sh.tile_log2, sh.wg, np, tiles, grid, p_small, flags,
+                                       (uint64_t)(uintptr_t)c->d_rec, (uint64_t)(uintptr_t)c->d_wins,
+                                       (uint64_t)(uintptr_t)c->d_out, (uint64_t)(uintptr_t)c->d_r0,
+                                       (uint64_t)(uintptr_t)c->d_rec_lo, (uint64_t)(uintptr_t)c->d_rec_hi};
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+        if (g.key == key) exec = g.exec;
+    if (!exec) {
+        if (c->graphs.size() >= GRAPH_CACHE) {
+            (void)hipGraphExecDestroy(c->graphs.front().exec);
+            c->graphs.erase(c->graphs.begin());
+        }
         hipGraph_t g = nullptr;
         HCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t e = launch_fill(s, c->d_slow_ctl, 2, 0);
         if (e == hipSuccess) e = launch_fill(s, c->d_counter, 1, 0);
-        if (wheel) {
-            if (e == hipSuccess) e = launch_residues_wheel(s, T, c->d_wins, nwin, max_np, c->d_r0);
+        // the residue launches cover the whole table: a piece's own count (read on the device)
+        // bounds the work, so chunks anywhere in the range share the graph
+        if (sh.wheel) {
+            if (e == hipSuccess) e = launch_residues_wheel(s, T, c->d_wins, np, c->nsp, c->d_r0);
             if (e == hipSuccess)
-                e = launch_wheel(s, T, mp, c->d_wins, nwin, tiles, cls_len, (uint32_t)grid, wg_per_cu, c->d_counter,
-                                 sl, c->d_rec, c->d_r0);
+                e = launch_wheel(s, T, mp, c->d_wins, np, tiles, cls_len, (uint32_t)grid, sh.wg, c->d_counter, sl,
+                                 c->d_rec, c->d_r0);
         } else {
-            if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, nwin, max_np, c->d_r0);
+            if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, np, c->nsp, c->d_r0);
             if (e == hipSuccess)
-                e = launch_fused(s, T, mp, c->d_wins, nwin, tiles, te, (uint32_t)grid, c->d_counter, verified_dev,
-                                 sl, c->d_rec, c->d_r0);
+                e = launch_fused(s, T, mp, c->d_wins, np, tiles, te, (uint32_t)grid, c->d_counter, sl, c->d_rec,
+                                 c->d_r0);
         }
-        if (e == hipSuccess) e = launch_slow(s, T, mp, sl, c->d_slow_res, c->d_wins, verified_dev, nullptr, 0);
+        if (e == hipSuccess) e = launch_slow(s, T, mp, sl, c->d_slow_res, c->d_wins, nullptr, nullptr, 0);
         if (e == hipSuccess)
-            e = launch_reduce(s, mp, c->d_rec, c->d_rec_lo, c->d_rec_hi, nwin, sl, c->d_slow_res, c->d_out);
+            e = launch_reduce(s, mp, c->d_rec, c->d_rec_lo, c->d_rec_hi, np, sl, c->d_slow_res, c->d_out);
         hipError_t e2 = hipStreamEndCapture(s, &g);
-        if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); HCK(c, e); }
+        if (e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            HCK(c, e);
+        }
         HCK(c, e2);
-        hipError_t e3 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        hipError_t e3 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         HCK(c, e3);
-        c->graph_key = key;
+        c->graphs.push_back(gb_ctx::Graph{key, exec});
     }
     HCK(c, hipEventRecord(c->ev0, s));
-    HCK(c, hipGraphLaunch(c->graph, s));
+    HCK(c, hipGraphLaunch(exec, s));
     HCK(c, hipEventRecord(c->ev1, s));
-    HCK(c, hipMemcpyAsync(c->h_out, c->d_out, nwin * sizeof(StatRec), hipMemcpyDeviceToHost, s));
+    HCK(c, hipMemcpyAsync(c->h_out, c->d_out, np * sizeof(StatRec), hipMemcpyDeviceToHost, s));
     HCK(c, hipMemcpyAsync(c->h_ctl, c->d_slow_ctl, 16, hipMemcpyDeviceToHost, s));
     HCK(c, hipStreamSynchronize(s));
     float f = 0;
     HCK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
-    if (ms) *ms = f;
-    const uint32_t* ctl = reinterpret_cast<const uint32_t*>(c->h_ctl);
-    if (ctl[2]) return fail(c, GB_ERANGE, "slow-path list overflow (more than 2^20 n with p_min beyond the window)");
+    if (ms) *ms += f;
+    *overflow = reinterpret_cast<const uint32_t*>(c->h_ctl)[2] != 0;
+    return GB_OK;
+}
+
+// Runs pieces of one shape in launches of at most CHUNK_TILES tiles and LAUNCH_PIECES pieces,
+// merging each piece's statistics into out[owner].  A launch whose slow list overflowed runs
+// again with its pieces halved (a single tile holds fewer n than the list).  stop_on_fail: stop
+// after the first launch with a failure.  *done (nullable) counts the evens finished.
+int run_pieces(gb_ctx* c, const Shape& sh, const std::vector<Piece>& pieces, uint64_t p_small, uint32_t flags,
+               gb_seg_stats* out, bool stop_on_fail, uint64_t* done, volatile uint64_t* progress, double* ms,
+               bool* failed) {
+    size_t i = 0;
+    while (i < pieces.size()) {
+        size_t j = i;
+        uint64_t tiles = 0;
+        while (j < pieces.size() && j - i < LAUNCH_PIECES) {
+            const uint64_t nt = piece_tiles(sh, pieces[j].start, pieces[j].evens);
+            if (j > i && tiles + nt > CHUNK_TILES) break;
+            tiles += nt;
+            ++j;
+        }
+        bool overflow = false;
+        int rc = run_launch(c, sh, &pieces[i], (uint32_t)(j - i), p_small, flags, ms, &overflow);
+        if (rc) return rc;
+        if (overflow) {
+            std::vector<Piece> halves;
+            for (size_t k = i; k < j; ++k) {
+                const uint64_t nt = piece_tiles(sh, pieces[k].start, pieces[k].evens);
+                if (nt == 1 && j - i == 1)
+                    return fail(c, GB_ERANGE, "slow-path list overflow within one tile");
+                cut_pieces(sh, pieces[k].start, pieces[k].start + 2 * (pieces[k].evens - 1), pieces[k].owner,
+                           std::max<uint64_t>(1, (nt + 1) / 2), halves);
+            }
+            // one launch per half: each holds at most half the tiles that overflowed
+            for (const Piece& h : halves) {
+                rc = run_pieces(c, sh, std::vector<Piece>{h}, p_small, flags, out, stop_on_fail, done, progress, ms,
+                                failed);
+                if (rc || (*failed && stop_on_fail)) return rc;
+            }
+            i = j;
+            continue;
+        }
+        for (size_t k = i; k < j; ++k) {
+            gb_seg_stats one{};
+            to_stats(c->h_out[k - i], 0, &one);
+            add_stats(&out[pieces[k].owner], one);
+            if (one.failures) *failed = true;
+            if (done) *done += one.evens;
+        }
+        if (progress && done) *progress = *done;
+        if (*failed && stop_on_fail) return GB_OK;
+        i = j;
+    }
+    return GB_OK;
+}
+
+// How gb_verify_range's batched form cuts [start, limit]: (shape, pieces) per kind, ascending.
+struct Part {
+    Shape sh;
+    std::vector<Piece> pieces;
+};
+
+int plan_batched(gb_ctx* c, uint64_t start, uint64_t limit, const gb_tuning& t, const gb_tuning* auto_pick,
+                 std::vector<Part>& parts) {
+    std::vector<std::pair<bool, std::pair<uint64_t, uint64_t>>> kinds;
+    split_kinds(start, limit, kinds);
+    // the autotuner's pick serves the part it was timed on: the larger one
+    size_t dom = 0;
+    for (size_t k = 1; k < kinds.size(); ++k)
+        if (kinds[k].second.second - kinds[k].second.first > kinds[dom].second.second - kinds[dom].second.first)
+            dom = k;
+    for (size_t k = 0; k < kinds.size(); ++k) {
+        Part P;
+        const bool wheel = kinds[k].first;
+        if (auto_pick && k == dom) P.sh = shape_of(wheel, auto_pick->tile_log2, auto_pick->wg_per_cu);
+        else if (auto_pick) P.sh = shape_of(wheel, 0, 0);
+        else P.sh = shape_of(wheel, t.tile_log2, t.wg_per_cu);
+        int rc = check_shape(c, P.sh);
+        if (rc) return rc;
+        cut_pieces(P.sh, kinds[k].second.first, kinds[k].second.second, 0, CHUNK_TILES, P.pieces);
+        parts.push_back(std::move(P));
+    }
     return GB_OK;
 }
 
 // Plain form over [start, limit]: one run_segment per seg evens, stopping at the first failure.
-static int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, const MarkParams& mp,
+int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, const MarkParams& mp,
                      gb_seg_stats* tot, volatile uint64_t* progress) {
     uint64_t done = 0;
     for (uint64_t a = start;;) {
@@ -734,6 +905,10 @@ static int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, co
     return GB_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 
 int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uint32_t form, uint32_t flags,
                 gb_tuning* best, double* evens_per_s) {
@@ -752,13 +927,29 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
             return GB_OK;
         }
     HCK(c, hipSetDevice(c->device));
+    // the batched form is timed on the part of the range its larger kernel serves (the wheel up to
+    // WHEEL_N_MAX, the fused kernel above), exactly as gb_verify_range will split it
+    std::vector<std::pair<bool, std::pair<uint64_t, uint64_t>>> kinds;
+    split_kinds(start, limit, kinds);
+    bool wheel = kinds[0].first;
+    uint64_t a = start, b = limit;
+    if (form == GB_FORM_BATCHED) {
+        size_t dom = 0;
+        for (size_t k = 1; k < kinds.size(); ++k)
+            if (kinds[k].second.second - kinds[k].second.first > kinds[dom].second.second - kinds[dom].second.first)
+                dom = k;
+        wheel = kinds[dom].first;
+        a = kinds[dom].second.first;
+        b = kinds[dom].second.second;
+    }
+    const Shape dsh = shape_of(wheel, 0, 0);
     gb_tuning def{};
     def.form = form;
     def.flags = flags;
-    def.tile_log2 = limit <= WHEEL_N_MAX ? WHEEL_TILE_LOG2 : DEFAULT_TILE_LOG2;
-    def.wg_per_cu = limit <= WHEEL_N_MAX ? 2 : 1;
+    def.tile_log2 = dsh.tile_log2;
+    def.wg_per_cu = dsh.wg;
     def.seg_evens = std::min<uint64_t>(DEFAULT_SEG, c->cfg.max_seg_evens);
-    const uint64_t evens = (limit - start) / 2 + 1;
+    const uint64_t evens = (b - a) / 2 + 1;
     // four sample windows at 1/8, 3/8, 5/8 and 7/8 of the range: the sieve's cost grows with q
     const uint64_t win = std::min<uint64_t>(evens / 4, form == GB_FORM_BATCHED ? (1ull << 30) : (1ull << 29));
     if (win < (1ull << 22)) {
@@ -770,21 +961,18 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
     for (int k = 0; k < 4; ++k) {
         const uint64_t mid = (evens / 8) * (2 * k + 1);
         const uint64_t first = std::min(mid - std::min(mid, win / 2), evens - win);
-        sample.push_back(WinDesc{start + 2 * first, win, 0, 0, 0, 0, 0, 0, 0});
+        sample.push_back(WinDesc{a + 2 * first, win, 0, 0, 0, 0, 0, 0, 0});
     }
     std::vector<gb_tuning> cand;
     if (form == GB_FORM_BATCHED) {
         for (uint32_t tl = 16; tl <= 20; ++tl) {
             ItemTable it;
-            rc = build_items(c, (1u << tl) + HALO + WIN_PAD, &it);
+            rc = wheel ? build_items(c, (1u << (tl - 1)) + WH_HALO + 64, &it, true)
+                       : build_items(c, (1u << tl) + HALO + WIN_PAD, &it);
             if (rc) return rc;
-            ItemTable wt;
-            rc = build_items(c, (1u << (tl - 1)) + WH_HALO + 64, &wt, true);
-            if (rc) return rc;
-            const bool wheel = limit <= WHEEL_N_MAX;
             for (uint32_t wg = 1; wg <= 2; ++wg) {
-                const size_t lds = wheel ? wheel_lds_bytes(wt.n_med, wg) : fused_lds_bytes(1u << tl, it.n_med);
-                if (lds * wg > LDS_PER_CU || (wheel && wg == 2 && (1u << (tl - 1)) > WH_CLASS_HALF)) continue;
+                const size_t lds = wheel ? wheel_lds_bytes(it.n_med, wg) : fused_lds_bytes(1u << tl, it.n_med);
+                if (lds * wg > LDS_PER_CU || check_shape(nullptr, Shape{wheel, tl, wg}) != GB_OK) continue;
                 gb_tuning t = def;
                 t.tile_log2 = tl;
                 t.wg_per_cu = wg;
@@ -808,8 +996,13 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
         for (int rep = 0; rep < 3; ++rep) {   // the first run of a candidate builds its schedule and graph
             const auto t0 = std::chrono::steady_clock::now();
             if (form == GB_FORM_BATCHED) {
+                const Shape sh{wheel, t.tile_log2, t.wg_per_cu};
+                std::vector<Piece> pcs;
+                for (const WinDesc& w : sample) cut_pieces(sh, w.start, w.start + 2 * (w.evens - 1), 0, CHUNK_TILES, pcs);
+                gb_seg_stats sink{};
                 double ms = 0;
-                rc = run_batched(c, sample, p_small, flags, t.tile_log2, t.wg_per_cu, nullptr, &ms);
+                bool failed = false;
+                rc = run_pieces(c, sh, pcs, p_small, flags, &sink, false, nullptr, nullptr, &ms, &failed);
                 if (rc) return rc;
             } else {
                 for (const WinDesc& w : sample) {
@@ -833,41 +1026,95 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
     return GB_OK;
 }
 
+// The tuning a range call runs with: the caller's, or with GB_TUNE_AUTO the autotuner's pick
+// (*auto_pick set; it applies to the part of the range it was timed on).
+static int resolve_tuning(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, const gb_tuning* tun,
+                          gb_tuning* t, gb_tuning* pick, bool* have_pick) {
+    *t = tun ? *tun : gb_tuning{};
+    *have_pick = false;
+    if (t->form != GB_FORM_PLAIN && t->form != GB_FORM_BATCHED) return fail(c, GB_EINVAL, "unknown form");
+    if (t->flags & GB_TUNE_AUTO) {
+        int rc = gb_autotune(c, start, limit, p_small, t->form, t->flags, pick, nullptr);
+        if (rc) return rc;
+        *have_pick = true;
+        t->seg_evens = pick->seg_evens;
+    }
+    return GB_OK;
+}
+
+int gb_range_plan(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, const gb_tuning* tun, gb_chunk* out,
+                  uint64_t cap, uint64_t* count) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (!count) return fail(c, GB_EINVAL, "gb_range_plan: count is NULL");
+    *count = 0;
+    int rc = normalise_range(c, &start, &limit);
+    if (rc) return rc;
+    HCK(c, hipSetDevice(c->device));
+    gb_tuning t{}, pick{};
+    bool have_pick = false;
+    rc = resolve_tuning(c, start, limit, p_small, tun, &t, &pick, &have_pick);
+    if (rc) return rc;
+    uint64_t n = 0;
+    auto emit = [&](const gb_chunk& ch) {
+        if (out && n < cap) out[n] = ch;
+        ++n;
+    };
+    if (t.form == GB_FORM_PLAIN) {
+        const uint64_t seg = t.seg_evens ? t.seg_evens : std::min<uint64_t>(DEFAULT_SEG, c->cfg.max_seg_evens);
+        if (seg > c->cfg.max_seg_evens) return fail(c, GB_ERANGE, "seg_evens above max_seg_evens");
+        for (uint64_t a = start;;) {
+            const uint64_t left = (limit - a) / 2 + 1, cnt = std::min(left, seg);
+            emit(gb_chunk{a, a + 2 * (cnt - 1), 0, GB_KERNEL_PLAIN, 0, 0, 0});
+            if (cnt == left) break;
+            a += 2 * cnt;
+        }
+    } else {
+        std::vector<Part> parts;
+        rc = plan_batched(c, start, limit, t, have_pick ? &pick : nullptr, parts);
+        if (rc) return rc;
+        for (const Part& P : parts)
+            for (const Piece& pc : P.pieces)
+                emit(gb_chunk{pc.start, pc.start + 2 * (pc.evens - 1), piece_tiles(P.sh, pc.start, pc.evens),
+                              P.sh.wheel ? GB_KERNEL_WHEEL : GB_KERNEL_FUSED, P.sh.tile_log2, P.sh.wg, 0});
+    }
+    *count = n;
+    return GB_OK;
+}
+
 int gb_verify_range(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, const gb_tuning* tun,
                     gb_seg_stats* st, volatile uint64_t* progress) {
     if (!c) return GB_EINVAL;
     c->err.clear();
-    int rc0 = normalise_range(c, &start, &limit);
-    if (rc0) return rc0;
+    int rc = normalise_range(c, &start, &limit);
+    if (rc) return rc;
     HCK(c, hipSetDevice(c->device));
-    gb_tuning t{};
-    if (tun) t = *tun;
-    if (t.flags & GB_TUNE_AUTO) {
-        gb_tuning a{};
-        rc0 = gb_autotune(c, start, limit, p_small, t.form, t.flags, &a, nullptr);
-        if (rc0) return rc0;
-        t.tile_log2 = a.tile_log2;
-        t.wg_per_cu = a.wg_per_cu;
-        t.seg_evens = a.seg_evens;
-    }
-    const uint64_t evens = (limit - start) / 2 + 1;
+    gb_tuning t{}, pick{};
+    bool have_pick = false;
+    rc = resolve_tuning(c, start, limit, p_small, tun, &t, &pick, &have_pick);
+    if (rc) return rc;
     gb_seg_stats tot{};
     if (t.form == GB_FORM_BATCHED) {
-        std::vector<WinDesc> w(1);
-        w[0] = WinDesc{start, evens, 0, 0, 0, 0, 0, 0, 0};
-        double ms = 0;
-        int rc = run_batched(c, w, p_small, t.flags & GB_WANT_PMIN, t.tile_log2, t.wg_per_cu, nullptr, &ms);
+        std::vector<Part> parts;
+        rc = plan_batched(c, start, limit, t, have_pick ? &pick : nullptr, parts);
         if (rc) return rc;
-        to_stats(c->h_out[0], ms, &tot);
-        if (progress) *progress = evens;
+        uint64_t done = 0;
+        double ms = 0;
+        bool failed = false;
+        for (const Part& P : parts) {
+            rc = run_pieces(c, P.sh, P.pieces, p_small, t.flags & GB_WANT_PMIN, &tot, true, &done, progress, &ms,
+                            &failed);
+            if (rc) return rc;
+            if (failed) break;   // the reference stops at its first failure (goldbach.cu:420,476-480)
+        }
+        tot.kernel_ms = ms;
         if (st) *st = tot;
         return GB_OK;
     }
-    if (t.form != GB_FORM_PLAIN) return fail(c, GB_EINVAL, "unknown form");
     uint64_t seg = t.seg_evens ? t.seg_evens : std::min<uint64_t>(DEFAULT_SEG, c->cfg.max_seg_evens);
     if (seg > c->cfg.max_seg_evens) return fail(c, GB_ERANGE, "seg_evens above max_seg_evens");
     const MarkParams mp = mark_params(c, p_small, t.flags);
-    int rc = run_plain(c, start, limit, seg, mp, &tot, progress);
+    rc = run_plain(c, start, limit, seg, mp, &tot, progress);
     if (rc) return rc;
     if (st) *st = tot;
     return GB_OK;
@@ -878,23 +1125,34 @@ int gb_verify_windows(gb_ctx* c, const gb_window* windows, uint32_t count, uint6
     if (!c) return GB_EINVAL;
     c->err.clear();
     if (!windows || count == 0) return fail(c, GB_EINVAL, "no windows");
-    std::vector<WinDesc> w(count);
     for (uint32_t i = 0; i < count; ++i) {
         const uint64_t a = windows[i].start, e = windows[i].evens;
         if (a < 4 || (a & 1) || e == 0 || e - 1 > (~0ull - 1 - a) / 2)
             return fail(c, GB_EINVAL, "window " + std::to_string(i) + " outside [4, 2^64-2] or not even");
-        w[i] = WinDesc{a, e, 0, 0, 0, 0, 0, 0, 0};
     }
     HCK(c, hipSetDevice(c->device));
+    // each window is split where the wheel stops and cut on its kernel's tile grid; the pieces of
+    // one kind share launches
+    const Shape shw = shape_of(true, 0, 0), shf = shape_of(false, 0, 0);
+    std::vector<Piece> pw, pf;
+    for (uint32_t i = 0; i < count; ++i) {
+        std::vector<std::pair<bool, std::pair<uint64_t, uint64_t>>> kinds;
+        const uint64_t a = windows[i].start;
+        split_kinds(a, a + 2 * (windows[i].evens - 1), kinds);
+        for (auto& k : kinds)
+            cut_pieces(k.first ? shw : shf, k.second.first, k.second.second, i, CHUNK_TILES, k.first ? pw : pf);
+    }
+    std::vector<gb_seg_stats> per(count);
     double ms = 0;
-    int rc = run_batched(c, w, p_small, flags, 0, 0, nullptr, &ms);
+    bool failed = false;
+    int rc = run_pieces(c, shw, pw, p_small, flags & GB_WANT_PMIN, per.data(), false, nullptr, nullptr, &ms, &failed);
+    if (rc) return rc;
+    rc = run_pieces(c, shf, pf, p_small, flags & GB_WANT_PMIN, per.data(), false, nullptr, nullptr, &ms, &failed);
     if (rc) return rc;
     gb_seg_stats tot{};
     for (uint32_t i = 0; i < count; ++i) {
-        gb_seg_stats one{};
-        to_stats(c->h_out[i], 0, &one);
-        if (per_window) per_window[i] = one;
-        add_stats(&tot, one);
+        if (per_window) per_window[i] = per[i];
+        add_stats(&tot, per[i]);
     }
     tot.kernel_ms = ms;
     if (total) *total = tot;
@@ -925,14 +1183,14 @@ int gb_count_partitions(gb_ctx* c, uint64_t start, uint64_t count, uint64_t* cou
     const Tables T = make_tables(c, it);
     uint64_t *d_P = nullptr, *d_R = nullptr, *d_cnt = nullptr;
     auto release = [&]() {
-        dfree(d_P);
-        dfree(d_R);
-        dfree(d_cnt);
+        dfree(c, d_P);
+        dfree(c, d_R);
+        dfree(c, d_cnt);
     };
-    if (dalloc(&d_P, pwords) != hipSuccess || dalloc(&d_R, pwords) != hipSuccess ||
-        dalloc(&d_cnt, count) != hipSuccess) {
+    if (dalloc(c, &d_P, pwords) != hipSuccess || dalloc(c, &d_R, pwords) != hipSuccess ||
+        dalloc(c, &d_cnt, count) != hipSuccess) {
         release();
-        return fail(c, GB_ENOMEM, "partition counts: device allocation failed");
+        return hip_fail(c, hipErrorOutOfMemory, __FILE__, __LINE__);
     }
     rc = ensure_wins(c, 1);
     if (rc == GB_OK) {
@@ -963,16 +1221,24 @@ int gb_count_partitions(gb_ctx* c, uint64_t start, uint64_t count, uint64_t* cou
 int gb_min_partition_batch(gb_ctx* c, const uint64_t* n, uint32_t count, uint64_t* p_out) {
     if (!c) return GB_EINVAL;
     c->err.clear();
+    return gbh::min_partition_bounded(c, n, count, c->cfg.phase2_limit, p_out);
+}
+
+}  // extern "C"
+
+namespace gbh {
+
+int min_partition_bounded(gb_ctx* c, const uint64_t* n, uint32_t count, uint64_t bound, uint64_t* p_out) {
     if (count == 0) return GB_OK;
     if (!n || !p_out) return fail(c, GB_EINVAL, "NULL buffer");
     HCK(c, hipSetDevice(c->device));
     if (count > c->q_cap) {
-        dfree(c->d_qn);
-        dfree(c->d_qp);
+        dfree(c, c->d_qn);
+        dfree(c, c->d_qp);
         c->d_qn = c->d_qp = nullptr;
         c->q_cap = 0;
-        HCK(c, dalloc(&c->d_qn, count));
-        HCK(c, dalloc(&c->d_qp, count));
+        HCK(c, dalloc(c, &c->d_qn, count));
+        HCK(c, dalloc(c, &c->d_qp, count));
         c->q_cap = count;
     }
     ItemTable it;
@@ -980,10 +1246,10 @@ int gb_min_partition_batch(gb_ctx* c, const uint64_t* n, uint32_t count, uint64_
     if (rc) return rc;
     const Tables T = make_tables(c, it);
     HCK(c, hipMemcpyAsync(c->d_qn, n, count * 8ull, hipMemcpyHostToDevice, c->stream));
-    HCK(c, launch_min_partition(c->stream, T, c->d_qn, count, c->cfg.phase2_limit, c->d_qp));
+    HCK(c, launch_min_partition(c->stream, T, c->d_qn, count, bound, c->d_qp));
     HCK(c, hipMemcpyAsync(p_out, c->d_qp, count * 8ull, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
     return GB_OK;
 }
 
-}  // extern "C"
+}  // namespace gbh

@@ -52,13 +52,14 @@ uint32_t trimmed(const uint32_t* w, uint32_t n) {
     return n;
 }
 
+// A device buffer for one call, counted against the context's memory budget.
 template <typename T>
 struct DevBuf {
+    gb_ctx* c;
     T* p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t count) { return hipMalloc(reinterpret_cast<void**>(&p), (count ? count : 1) * sizeof(T)); }
+    explicit DevBuf(gb_ctx* ctx) : c(ctx) {}
+    ~DevBuf() { dfree(c, p); }
+    hipError_t alloc(size_t count) { return dalloc(c, &p, count); }
 };
 
 }  // namespace
@@ -141,9 +142,9 @@ int gb_big_min_partition(gb_ctx* c, const uint32_t* n_words, uint32_t nwords, ui
         // below 2^64 the strong test to bases 2..37 is exact: the 64-bit path gives the same p
         const uint64_t n = (uint64_t)n_words[0] | (L == 2 ? (uint64_t)n_words[1] << 32 : 0);
         uint64_t p = 0;
-        const int rc = gb_min_partition_batch(c, &n, 1, &p);
+        const int rc = min_partition_bounded(c, &n, 1, p_max, &p);
         if (rc) return rc;
-        *p_out = p <= p_max ? p : 0;
+        *p_out = p;
         if (st) *st = stats;
         return GB_OK;
     }
@@ -154,8 +155,8 @@ int gb_big_min_partition(gb_ctx* c, const uint32_t* n_words, uint32_t nwords, ui
     const std::vector<uint32_t> primes = odd_primes(p_max);
     const std::vector<uint32_t> ell = odd_primes(FILTER_BOUND);
     const uint64_t bad_words = p_max / 64 + 1;
-    DevBuf<uint32_t> d_n, d_ell, d_r, d_bad, d_pass;
-    DevBuf<gbk::BigTask> d_tasks;
+    DevBuf<uint32_t> d_n(c), d_ell(c), d_r(c), d_bad(c), d_pass(c);
+    DevBuf<gbk::BigTask> d_tasks(c);
     // one batch = the tests the device holds at once (LDS and 2048 threads per CU); a
     // 1024-thread test already fills its CU (two per CU took twice as long as one), so those
     // run one per CU and the batches stay short
@@ -168,7 +169,7 @@ int gb_big_min_partition(gb_ctx* c, const uint32_t* n_words, uint32_t nwords, ui
     if (d_n.alloc(L) != hipSuccess || d_ell.alloc(ell.size()) != hipSuccess || d_r.alloc(ell.size()) != hipSuccess ||
         d_bad.alloc(bad_words) != hipSuccess || d_tasks.alloc(std::max<uint32_t>(batch, MAX_ROUNDS)) != hipSuccess ||
         d_pass.alloc(std::max<uint32_t>(batch, MAX_ROUNDS)) != hipSuccess)
-        return fail(c, GB_ENOMEM, "gb_big_min_partition: device allocation failed");
+        return hip_fail(c, hipErrorOutOfMemory, __FILE__, __LINE__);
     HCK(c, hipMemcpyAsync(d_n.p, n_words, L * 4, hipMemcpyHostToDevice, s));
     HCK(c, hipMemcpyAsync(d_ell.p, ell.data(), ell.size() * 4, hipMemcpyHostToDevice, s));
     HCK(c, hipMemsetAsync(d_bad.p, 0, bad_words * 4, s));

@@ -7,6 +7,7 @@
 
 #include <atomic>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -36,6 +37,11 @@ struct gb_ctx {
     gb_config cfg{};
     int cus = 0;
     uint64_t total_mem = 0;
+    // device memory the context holds, against its budget (every hipMalloc goes through gbh::dalloc)
+    uint64_t budget = 0;
+    uint64_t mem_used = 0;
+    bool budget_hit = false;
+    std::unordered_map<void*, uint64_t> held;
     // prime tables
     uint32_t* d_sp = nullptr;
     double* d_inv = nullptr;
@@ -76,11 +82,17 @@ struct gb_ctx {
     uint32_t q_cap = 0;
     // host staging (pinned)
     gbk::StatRec* h_out = nullptr;
+    gbk::WinDesc* h_wins = nullptr;    // a launch's pieces, copied to d_wins before it runs
+    uint64_t* h_lohi = nullptr;        // [rec_lo | rec_hi] of those pieces
     uint64_t* h_ctl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // cached batched graph
-    hipGraphExec_t graph = nullptr;
-    std::vector<uint64_t> graph_key;
+    // batched launches captured as graphs; a graph's kernels read their pieces from d_wins, so
+    // one graph serves every chunk of the same shape
+    struct Graph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
     // autotuner results: key (form, start, limit, p_small, flags) -> tuning
     std::vector<std::pair<std::vector<uint64_t>, gb_tuning>> tune_cache;
     // serving engines running on this context's tables
@@ -96,24 +108,43 @@ int fail(gb_ctx* c, int code, const std::string& msg);
 // even >= 4; GB_EINVAL when the range is empty.
 int normalise_range(gb_ctx* c, uint64_t* start, uint64_t* limit);
 
+// HIP status -> gb_status with a message (device out-of-memory and the context's own budget
+// both give GB_ENOMEM).
+int hip_fail(gb_ctx* c, hipError_t e, const char* file, int line);
+
 #define HCK(ctx, call)                                                                         \
     do {                                                                                       \
         hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return fail(ctx, GB_EDEVICE, std::string("HIP error at ") + __FILE__ + ":" +       \
-                                             std::to_string(__LINE__) + " -- " + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, __FILE__, __LINE__);                    \
     } while (0)
 
+// Device allocation counted against the context's memory budget (gb_config.mem_budget);
+// hipErrorOutOfMemory, without allocating, when it would pass the budget.
 template <typename T>
-hipError_t dalloc(T** p, uint64_t count) {
+hipError_t dalloc(gb_ctx* c, T** p, uint64_t count) {
     *p = nullptr;
     if (!count) count = 1;
-    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    const uint64_t bytes = count * sizeof(T);
+    if (bytes / sizeof(T) != count || c->mem_used + bytes > c->budget) {
+        c->budget_hit = true;
+        return hipErrorOutOfMemory;
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+    if (e == hipSuccess) {
+        c->mem_used += bytes;
+        c->held[*p] = bytes;
+    }
+    return e;
 }
+void dfree(gb_ctx* c, void* p);
 
 // Medium-prime schedule of the sieve for tiles of at most `tb` bits (cached on c).
 int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel = false);
 // Device table view of c with the schedule `it`.
 gbk::Tables make_tables(const gb_ctx* c, const ItemTable& it);
+
+// p_out[i] = smallest prime p <= bound with n[i] - p prime (0 if none): gb_min_partition_batch
+// with a bound other than the context's phase2_limit.
+int min_partition_bounded(gb_ctx* c, const uint64_t* n, uint32_t count, uint64_t bound, uint64_t* p_out);
 
 }  // namespace gbh

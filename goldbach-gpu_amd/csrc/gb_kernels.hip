@@ -743,8 +743,7 @@ __global__ __launch_bounds__(BS) void mark_segment_kernel(Tables T, MarkParams m
 template <int BS>
 __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, const WinDesc* __restrict__ wins,
                                                    uint32_t nwin, uint64_t ntiles, uint32_t tile_evens,
-                                                   uint64_t* __restrict__ tile_counter,
-                                                   uint64_t* __restrict__ verified, SlowList sl,
+                                                   uint64_t* __restrict__ tile_counter, SlowList sl,
                                                    StatRec* __restrict__ rec, const PrimeRes* __restrict__ pr) {
     // the window at a fixed LDS address (sized for the largest tile); dynamic
     // LDS after it: [medium first marks | stats | tile id]
@@ -775,8 +774,7 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
         const WinGeom g = window_geom(n0, (uint64_t)cnt + HALO + WIN_PAD);
         sieve_tile<BS>(win32, scratch, FUSED_MRQ, wsum, g.qv0, g.lead, 0, g.nbits, T, tid, pr + wd.r0off, wd.rbase);
         LaneStats st = lane_stats_zero();
-        mark_words<BS>(reinterpret_cast<const uint64_t*>(win32), n0, cnt, mp, T.sp,
-                       verified ? verified + wd.word0 + lt * (tile_evens / 64) : nullptr, nullptr, sl, lo, st,
+        mark_words<BS>(reinterpret_cast<const uint64_t*>(win32), n0, cnt, mp, T.sp, nullptr, nullptr, sl, lo, st,
                        tid);
         block_store<BS>(st, cnt, &rec[t], tid, red);
     }
@@ -1431,6 +1429,10 @@ __device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint6
     if (in && isp) {
         if (q <= T.p2_limit) {
             pass = is_prime_q(q, T);
+        } else if (q < 41) {
+            // a Phase-2 bound below 41 leaves small q here: small_factor would reject the
+            // primes 3..23 themselves and the strong tests need q > 37
+            pass = (0x20A08A28ACull >> q) & 1;
         } else if (!small_factor(q)) {
             pass = mr_test(q, 0, false);
         }
@@ -1440,7 +1442,7 @@ __device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint6
         const uint32_t L = __ffsll((unsigned long long)ball) - 1;
         const uint64_t qq = __shfl(q, L);
         bool ok = true;
-        if (qq > T.p2_limit && lane >= 1 && lane < mr_bases(qq)) ok = mr_test(qq, lane, mr_wide(qq));
+        if (qq > T.p2_limit && qq >= 41 && lane >= 1 && lane < mr_bases(qq)) ok = mr_test(qq, lane, mr_wide(qq));
         if (__all(ok)) {
             res = __shfl(p, L);
             return 1;
@@ -1571,13 +1573,13 @@ size_t fused_lds_bytes(uint32_t tile_evens, uint32_t n_med) {
 
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
-                        uint64_t* verified, SlowList sl, StatRec* rec, const PrimeRes* pr) {
+                        SlowList sl, StatRec* rec, const PrimeRes* pr) {
     if (tile_evens > FUSED_TILE_MAX || (tile_evens & 63)) return hipErrorInvalidValue;
     const size_t lds = fused_dyn_lds(T.n_med);
     hipError_t e = allow_lds(fused_kernel<FUSED_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fused_kernel<FUSED_BS>, dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin, ntiles,
-                       tile_evens, tile_counter, verified, sl, rec, pr);
+                       tile_evens, tile_counter, sl, rec, pr);
     return hipGetLastError();
 }
 

@@ -97,7 +97,7 @@ constexpr size_t LDS_PER_CU = 160 * 1024;
 // Fused persistent kernel over the tiles of `nwin` windows.
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
-                        uint64_t* verified, SlowList sl, StatRec* rec, const PrimeRes* pr);
+                        SlowList sl, StatRec* rec, const PrimeRes* pr);
 
 // Wheel (mod 6) form of the fused fast path: tiles of 3 * cls_len even n (cls_len per
 // class n mod 6), the window kept as two streams of q = 6k + 1 and q = 6k + 5.

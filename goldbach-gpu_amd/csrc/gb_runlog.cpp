@@ -176,7 +176,11 @@ int read_log(const char* path, bool* exists, gb_log_header* h, std::vector<gb_lo
         uint64_t a, b;
         if (r.chunk != recs->size() || r.chunk >= nch) break;
         chunk_bounds(*h, r.chunk, &a, &b);
-        if (r.first_n != a || r.last_n != b || r.evens != (b - a) / 2 + 1) break;
+        // a chunk holding a failure may stop short of its end (the range stops at the
+        // first failing segment or sub-chunk); every other chunk covers all its evens
+        const uint64_t full = (b - a) / 2 + 1;
+        if (r.first_n != a || r.last_n != b) break;
+        if (r.failures ? (r.evens == 0 || r.evens > full) : r.evens != full) break;
         recs->push_back(r);
         pos = e + 1;
         *valid_bytes = pos;

@@ -275,6 +275,13 @@ int main(int argc, char** argv) {
             std::cout << "Autotuned      : segments of " << tun.seg_evens << " evens";
         std::cout << " (" << std::scientific << std::setprecision(3) << rate << std::defaultfloat
                   << " evens/s on the samples)\n\n";
+        // the pick is cached on the context: the range call applies it to the part of the range it
+        // was timed on (a run log's chunks take it as given)
+        if (log_path.empty() && form == GB_FORM_BATCHED) {
+            tun.flags |= GB_TUNE_AUTO;
+            tun.tile_log2 = 0;
+            tun.wg_per_cu = 0;
+        }
     }
     gb_seg_stats st{};
     gb_log_info li{};

@@ -60,7 +60,9 @@ typedef struct gb_config {
     uint64_t max_n;          /* largest even n any call will touch (default 2^64-2) */
     uint64_t max_seg_evens;  /* largest plain segment, in evens (default 2^27) */
     uint64_t phase2_limit;   /* Phase-2 prime bound (goldbach.cu:316, default 1e8) */
-    uint64_t mem_budget;     /* device bytes the context may hold (default: all but 1 GiB) */
+    uint64_t mem_budget;     /* device bytes the context may hold (default: all but 1 GiB); every
+                                device allocation of the context counts against it and a call
+                                that would pass it returns GB_ENOMEM */
     uint32_t serve_slots;    /* serving slots (default 65536) */
     uint32_t reserved;
 } gb_config;
@@ -121,9 +123,10 @@ int gb_verify_segment(gb_ctx *ctx, uint64_t seg_start, uint64_t seg_even_count, 
 
 typedef struct gb_tuning {
     uint32_t form;          /* GB_FORM_* */
-    uint32_t tile_log2;     /* batched: tile size 2^tile_log2 (0: 19 with wg_per_cu 2 when every n is
-                               below 2^44 - 2^24, where tiles use the mod-6 wheel and hold
-                               1.5 * 2^tile_log2 evens; otherwise 20) */
+    uint32_t tile_log2;     /* batched: tile size 2^tile_log2 (0 with wg_per_cu 0: 19 with two
+                               workgroups per CU for the part at or below 2^44 - 2^24, where tiles
+                               use the mod-6 wheel and hold 1.5 * 2^tile_log2 evens, and 20 with
+                               one above; a value given applies to both parts) */
     uint64_t seg_evens;     /* plain: evens per segment (0 = 2^27) */
     uint32_t wg_per_cu;     /* batched: persistent workgroups per CU (0 = default; 2 needs tile_log2 <= 19) */
     uint32_t flags;         /* GB_WANT_PMIN | GB_TUNE_AUTO */
@@ -145,11 +148,43 @@ int gb_autotune(gb_ctx *ctx, uint64_t start, uint64_t limit, uint64_t p_small, u
 
 /* All even n in [start, limit] (start even >= 4 after the reference's
  * adjustments, goldbach.cu:594,604-609).  Replaces main's worker spawn and
- * the segment queue (goldbach.cu:689, 761-773, 378-500).  progress_evens
- * (nullable) is updated as work completes, for a progress printer. */
+ * the segment queue (goldbach.cu:689, 761-773, 378-500).
+ *   plain form:   one segment of seg_evens after another, one host sync each.
+ *   batched form: the range is split at 2^44 - 2^24 (below, tiles of the
+ *                 mod-6 wheel kernel; above, the odd-only fused kernel with
+ *                 exact 64-bit tests) and each part is cut into launches of
+ *                 at most 2^17 tiles, so the device memory a call holds does
+ *                 not grow with the range (the reference's fixed per-segment
+ *                 buffers, goldbach.cu:378-417).
+ * Both forms stop after the first segment or launch holding a failure, as
+ * the reference stops its workers (goldbach.cu:420,476-480); stats then
+ * cover the evens examined up to there.  progress_evens (nullable) is
+ * updated after every segment or launch, for a progress printer
+ * (goldbach.cu:702-758). */
 int gb_verify_range(gb_ctx *ctx, uint64_t start, uint64_t limit, uint64_t p_small,
                     const gb_tuning *tuning, gb_seg_stats *stats,
                     volatile uint64_t *progress_evens);
+
+/* How gb_verify_range cuts [start, limit] with this tuning, without running
+ * it (with GB_TUNE_AUTO the autotuner runs first).  Each gb_chunk is one
+ * plain segment or one batched launch; *count gets their number and the
+ * first cap are written to out (nullable).  No counterpart in the
+ * reference, whose cut is fixed (goldbach.cu:422-431). */
+#define GB_KERNEL_PLAIN 0u   /* sieve + marking launches of one plain segment */
+#define GB_KERNEL_WHEEL 1u   /* persistent mod-6 wheel kernel (every n <= 2^44 - 2^24) */
+#define GB_KERNEL_FUSED 2u   /* persistent odd-only kernel (n above that) */
+
+typedef struct gb_chunk {
+    uint64_t first_n, last_n;   /* even n covered */
+    uint64_t tiles;             /* batched: tiles of the launch (0 for a plain segment) */
+    uint32_t kernel;            /* GB_KERNEL_* */
+    uint32_t tile_log2;         /* batched: fused tiles of 2^t evens, wheel tiles of 3 * 2^(t-1) */
+    uint32_t wg_per_cu;
+    uint32_t reserved;
+} gb_chunk;
+
+int gb_range_plan(gb_ctx *ctx, uint64_t start, uint64_t limit, uint64_t p_small, const gb_tuning *tuning,
+                  gb_chunk *out, uint64_t cap, uint64_t *count);
 
 /* ---- uneven high windows (q by pre-sieve and Baillie-PSW) -------------- */
 

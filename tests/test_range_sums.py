@@ -27,11 +27,16 @@ def test_fixture_shape():
 
 
 def test_records_are_the_ranges_maxima(orc):
-    got = {(r["argmax_n"], r["max_pmin"]) for r in fixtures()}
-    assert got == set(RECORDS)
-    for n, p in RECORDS:
+    # a range from 4 ends its running maximum at a published record; every range's argmax has
+    # exactly the stated minimal partition
+    rec = dict(RECORDS)
+    for r in fixtures():
+        n, p = r["argmax_n"], r["max_pmin"]
+        if r["start"] == 4 and r["limit"] <= 1.2e10:
+            assert rec.get(n) == p, (r["start"], r["limit"])
         assert orc.min_partition(n) == p
         assert orc.is_prime(p) and orc.is_prime(n - p)
+    assert {(r["argmax_n"], r["max_pmin"]) for r in fixtures()} >= set(RECORDS)
 
 
 @pytest.mark.parametrize("idx", [0])
