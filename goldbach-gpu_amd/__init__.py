@@ -37,7 +37,7 @@ EXPORTS = (
     "gb_min_partition_batch", "gb_autotune", "gb_count_partitions", "gb_verify_range_logged", "gb_log_read",
     "gb_big_min_partition", "gb_big_from_decimal", "gb_big_to_decimal",
     "gb_serve_open", "gb_serve_submit", "gb_serve_poll", "gb_serve_stats_get", "gb_serve_close",
-    "gb_serve_last_error", "gb_range_plan",
+    "gb_serve_last_error", "gb_range_plan", "gb_verify_segment_noalloc",
 )
 GB_KERNEL_PLAIN, GB_KERNEL_WHEEL, GB_KERNEL_FUSED = 0, 1, 2
 
@@ -151,6 +151,7 @@ def library() -> ctypes.CDLL:
                                           ctypes.POINTER(ctypes.c_int)]),
         "gb_sieve_window": (ctypes.c_int, [vp, u64, u64, vp]),
         "gb_verify_segment": (ctypes.c_int, [vp, u64, u64, u64, u32, vp, vp, ctypes.POINTER(GbSegStats)]),
+        "gb_verify_segment_noalloc": (ctypes.c_int, [vp, u64, u64, u64, u32, vp, vp, ctypes.POINTER(GbSegStats)]),
         "gb_verify_range": (ctypes.c_int, [vp, u64, u64, u64, ctypes.POINTER(GbTuning), ctypes.POINTER(GbSegStats),
                                            vp]),
         "gb_verify_windows": (ctypes.c_int, [vp, ctypes.POINTER(GbWindow), u32, u64, u32,
@@ -300,6 +301,18 @@ class Context:
         self._check(self._lib.gb_verify_segment(self._h, seg_start, count, p_small, flags,
                                                 _ptr(ver) if ver is not None else None,
                                                 _ptr(pm) if pm is not None else None, ctypes.byref(st)))
+        return SegmentResult(st.as_dict(), ver, pm)
+
+    def verify_segment_noalloc(self, seg_start: int, count: int, p_small: int, want_pmin: bool = False,
+                               want_verified: bool = True, want_pmin_array: bool = False) -> SegmentResult:
+        """verify_segment through gb_verify_segment_noalloc (static device scratch, count <= 2^20)."""
+        st = GbSegStats()
+        ver = np.zeros((count + 63) // 64, dtype=np.uint64) if want_verified else None
+        pm = np.zeros(count, dtype=np.uint32) if want_pmin_array else None
+        flags = GB_WANT_PMIN if (want_pmin or want_pmin_array) else 0
+        self._check(self._lib.gb_verify_segment_noalloc(self._h, seg_start, count, p_small, flags,
+                                                        _ptr(ver) if ver is not None else None,
+                                                        _ptr(pm) if pm is not None else None, ctypes.byref(st)))
         return SegmentResult(st.as_dict(), ver, pm)
 
     # -- whole range -------------------------------------------------------

@@ -39,6 +39,35 @@ constexpr uint64_t COUNT_N_MAX = 1ull << 36;             // partition counts: bi
 std::mutex g_open_mu;
 std::string g_open_err;
 
+// ---- memory of the allocation-free entry point (gb_verify_segment_noalloc) ----
+// One segment of at most NA_SEG_MAX evens: window bitset, verified map and p_min staging,
+// marking records, the window's sieving-prime entries (every table prime at most), one
+// descriptor, the result record and a slow list of NA_SLOW_CAP entries.
+constexpr uint64_t NA_SEG_MAX = 1ull << 20;
+constexpr uint32_t NA_SLOW_CAP = 1u << 14;
+constexpr uint64_t na_align(uint64_t x) { return (x + 255) & ~255ull; }
+constexpr uint64_t NA_WIN_WORDS = (NA_SEG_MAX + HALO + WIN_PAD + 63) / 64 + 1;
+constexpr uint64_t NA_VER_WORDS = (NA_SEG_MAX + 63) / 64;
+constexpr uint64_t NA_RECS = (NA_VER_WORDS + MARK_WPB - 1) / MARK_WPB;
+constexpr uint64_t NA_OFF_WIN = 0;
+constexpr uint64_t NA_OFF_VER = na_align(NA_OFF_WIN + NA_WIN_WORDS * 8);
+constexpr uint64_t NA_OFF_PMIN = na_align(NA_OFF_VER + NA_VER_WORDS * 8);
+constexpr uint64_t NA_OFF_REC = na_align(NA_OFF_PMIN + NA_SEG_MAX * 4);
+constexpr uint64_t NA_OFF_R0 = na_align(NA_OFF_REC + NA_RECS * sizeof(StatRec));
+constexpr uint64_t NA_OFF_WINS = na_align(NA_OFF_R0 + (uint64_t)SP_TABLE_CAP * sizeof(PrimeRes));
+constexpr uint64_t NA_OFF_LO = na_align(NA_OFF_WINS + sizeof(WinDesc));
+constexpr uint64_t NA_OFF_HI = na_align(NA_OFF_LO + 8);
+constexpr uint64_t NA_OFF_OUT = na_align(NA_OFF_HI + 8);
+constexpr uint64_t NA_OFF_SLOW_N = na_align(NA_OFF_OUT + sizeof(StatRec));
+constexpr uint64_t NA_OFF_SLOW_WIN = na_align(NA_OFF_SLOW_N + NA_SLOW_CAP * 8ull);
+constexpr uint64_t NA_OFF_SLOW_RES = na_align(NA_OFF_SLOW_WIN + NA_SLOW_CAP * 4ull);
+constexpr uint64_t NA_OFF_SLOW_CTL = na_align(NA_OFF_SLOW_RES + NA_SLOW_CAP * 8ull);
+constexpr uint64_t NA_SCRATCH_BYTES = na_align(NA_OFF_SLOW_CTL + 16);   // 9.9 MB
+static_assert(NA_SCRATCH_BYTES < (12ull << 20), "scratch stays modest");
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch;   // per device: the address of its copy of scratch
+
 void dhostfree(void* p) {
     if (p) (void)hipHostFree(p);
 }
@@ -48,6 +77,11 @@ uint32_t count_le(const std::vector<uint32_t>& sp, uint64_t x) {
 }
 
 }  // namespace
+
+// Each device holds its own copy, from the first use of the library on it to the end of the
+// process.  In host code `scratch` only names the symbol: its device address comes from
+// hipGetSymbolAddress, once per device (g_scratch), and only that address reaches kernels.
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
 
 namespace gbh {
 
@@ -369,18 +403,103 @@ int build_tables(gb_ctx* c) {
     return GB_OK;
 }
 
-// One plain segment already validated; results left in c->h_out[0].
+// Device buffers one plain segment runs on: the context's own (grown on demand) or the
+// allocation-free entry point's slices of `scratch`.
+struct SegBufs {
+    uint64_t* window;
+    uint64_t window_cap;   // words
+    uint64_t* verified;
+    uint64_t verified_cap; // words
+    uint32_t* pmin;
+    uint64_t pmin_cap;
+    StatRec* rec;
+    uint64_t rec_cap;
+    PrimeRes* r0;
+    uint64_t r0_cap;
+    WinDesc* wins;         // one descriptor
+    uint64_t* rec_lo;
+    uint64_t* rec_hi;
+    StatRec* out;
+    SlowList sl;
+    uint64_t* slow_res;
+    uint64_t* slow_ctl;
+};
+
+SegBufs ctx_bufs(const gb_ctx* c) {
+    return SegBufs{c->d_window, c->window_cap, c->d_verified, c->verified_cap, c->d_pmin, c->pmin_cap,
+                   c->d_rec, c->rec_cap, c->d_r0, c->r0_cap, c->d_wins, c->d_rec_lo, c->d_rec_hi, c->d_out,
+                   slow_list(c), c->d_slow_res, c->d_slow_ctl};
+}
+
+// Launch geometry of one plain segment: window words, verified words, marking records and the
+// window's sieving-prime entries (the descriptor is left in c->h_wins[0]).
+struct SegPlan {
+    WinGeom g;
+    uint64_t wwords, vwords, nblocks;
+    uint32_t np;
+};
+
+SegPlan plan_segment(gb_ctx* c, uint64_t A, uint64_t cnt) {
+    SegPlan sp;
+    sp.g = window_geom(A, cnt + HALO + WIN_PAD);
+    sp.wwords = ((uint64_t)sp.g.nbits + 63) / 64;
+    sp.vwords = (cnt + 63) / 64;
+    sp.nblocks = (sp.vwords + MARK_WPB - 1) / MARK_WPB;
+    WinDesc wd{A, cnt, 0, 0, 0, 0, 0, 0, 0};
+    set_residues(c, &wd, sp.g.qv0, geom_q_top(sp.g), 0);
+    c->h_wins[0] = wd;
+    sp.np = wd.np;
+    return sp;
+}
+
+// One plain segment on buffers that are already large enough; allocates nothing.  Results are
+// left in c->h_out[0].
+int run_segment_on(gb_ctx* c, const SegBufs& B, const SegPlan& sp, uint64_t A, uint64_t cnt, const MarkParams& mp,
+                   bool want_verified, bool want_pmin, double* ms) {
+    hipStream_t s = c->stream;
+    const ItemTable* it = nullptr;       // the plain tile schedule, built by gb_open
+    for (const ItemTable& t : c->items)
+        if (t.tb == PLAIN_TILE_BITS && !t.wheel) it = &t;
+    if (!it) return fail(c, GB_EDEVICE, "plain sieve schedule missing");
+    const Tables T = make_tables(c, *it);
+    if (sp.wwords > B.window_cap || sp.vwords > B.verified_cap || (want_pmin && cnt > B.pmin_cap) ||
+        sp.nblocks > B.rec_cap || sp.np > B.r0_cap)
+        return fail(c, GB_ERANGE, "segment larger than the buffers it runs on");
+    c->h_lohi[0] = 0;
+    c->h_lohi[1] = sp.nblocks;
+    HCK(c, hipMemcpyAsync(B.wins, c->h_wins, sizeof(WinDesc), hipMemcpyHostToDevice, s));
+    HCK(c, hipMemcpyAsync(B.rec_lo, &c->h_lohi[0], 8, hipMemcpyHostToDevice, s));
+    HCK(c, hipMemcpyAsync(B.rec_hi, &c->h_lohi[1], 8, hipMemcpyHostToDevice, s));
+    HCK(c, hipEventRecord(c->ev0, s));
+    HCK(c, launch_fill(s, B.slow_ctl, 2, 0));
+    HCK(c, launch_residues(s, T, B.wins, 1, sp.np, B.r0));
+    HCK(c, launch_sieve_window(s, T, sp.g.qv0, sp.g.lead, sp.g.nbits, PLAIN_TILE_BITS, B.window, 0, B.r0));
+    uint32_t nrec = 0;
+    HCK(c, launch_mark_segment(s, T, mp, A, cnt, B.window, sp.wwords, MARK_WPB, want_verified ? B.verified : nullptr,
+                               want_pmin ? B.pmin : nullptr, B.sl, B.rec, &nrec));
+    HCK(c, launch_slow(s, T, mp, B.sl, B.slow_res, B.wins, want_verified ? B.verified : nullptr,
+                       want_pmin ? B.pmin : nullptr, A));
+    HCK(c, launch_reduce(s, mp, B.rec, B.rec_lo, B.rec_hi, 1, B.sl, B.slow_res, B.out));
+    HCK(c, hipEventRecord(c->ev1, s));
+    HCK(c, hipMemcpyAsync(c->h_out, B.out, sizeof(StatRec), hipMemcpyDeviceToHost, s));
+    HCK(c, hipMemcpyAsync(c->h_ctl, B.slow_ctl, 16, hipMemcpyDeviceToHost, s));
+    HCK(c, hipStreamSynchronize(s));
+    float f = 0;
+    HCK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ms) *ms = f;
+    const uint32_t* ctl = reinterpret_cast<const uint32_t*>(c->h_ctl);
+    if (ctl[2])
+        return fail(c, GB_ERANGE, "slow-path list overflow (more than " + std::to_string(B.sl.cap) +
+                                      " n with p_min beyond the window)");
+    return GB_OK;
+}
+
+// One plain segment already validated, on the context's buffers (grown as needed).
 int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool want_verified, bool want_pmin,
                 double* ms) {
-    hipStream_t s = c->stream;
-    ItemTable it;
-    int rc = build_items(c, PLAIN_TILE_BITS, &it);
-    if (rc) return rc;
-    const Tables T = make_tables(c, it);
-    const WinGeom g = window_geom(A, cnt + HALO + WIN_PAD);
-    const uint64_t wwords = ((uint64_t)g.nbits + 63) / 64;
-    const uint64_t vwords = (cnt + 63) / 64;
-    if (wwords > c->window_cap || vwords > c->verified_cap) return fail(c, GB_ERANGE, "segment larger than max_seg_evens");
+    const SegPlan sp = plan_segment(c, A, cnt);
+    if (sp.wwords > c->window_cap || sp.vwords > c->verified_cap)
+        return fail(c, GB_ERANGE, "segment larger than max_seg_evens");
     if (want_pmin && cnt > c->pmin_cap) {
         dfree(c, c->d_pmin);
         c->d_pmin = nullptr;
@@ -388,41 +507,11 @@ int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool 
         HCK(c, dalloc(c, &c->d_pmin, cnt));
         c->pmin_cap = cnt;
     }
-    const uint64_t nblocks = (vwords + MARK_WPB - 1) / MARK_WPB;
-    rc = ensure_rec(c, nblocks);
+    int rc = ensure_rec(c, sp.nblocks);
     if (rc) return rc;
-    rc = ensure_wins(c, 1);
+    rc = ensure_r0(c, sp.np);
     if (rc) return rc;
-    WinDesc wd{A, cnt, 0, 0, 0, 0, 0, 0, 0};
-    set_residues(c, &wd, g.qv0, geom_q_top(g), 0);
-    rc = ensure_r0(c, wd.np);
-    if (rc) return rc;
-    uint64_t lohi[2] = {0, nblocks};
-    HCK(c, hipMemcpyAsync(c->d_wins, &wd, sizeof(wd), hipMemcpyHostToDevice, s));
-    HCK(c, hipMemcpyAsync(c->d_rec_lo, &lohi[0], 8, hipMemcpyHostToDevice, s));
-    HCK(c, hipMemcpyAsync(c->d_rec_hi, &lohi[1], 8, hipMemcpyHostToDevice, s));
-    SlowList sl = slow_list(c);
-    HCK(c, hipEventRecord(c->ev0, s));
-    HCK(c, launch_fill(s, c->d_slow_ctl, 2, 0));
-    HCK(c, launch_residues(s, T, c->d_wins, 1, wd.np, c->d_r0));
-    HCK(c, launch_sieve_window(s, T, g.qv0, g.lead, g.nbits, PLAIN_TILE_BITS, c->d_window, 0, c->d_r0));
-    uint32_t nrec = 0;
-    HCK(c, launch_mark_segment(s, T, mp, A, cnt, c->d_window, wwords, MARK_WPB,
-                               want_verified ? c->d_verified : nullptr, want_pmin ? c->d_pmin : nullptr, sl,
-                               c->d_rec, &nrec));
-    HCK(c, launch_slow(s, T, mp, sl, c->d_slow_res, c->d_wins, want_verified ? c->d_verified : nullptr,
-                       want_pmin ? c->d_pmin : nullptr, A));
-    HCK(c, launch_reduce(s, mp, c->d_rec, c->d_rec_lo, c->d_rec_hi, 1, sl, c->d_slow_res, c->d_out));
-    HCK(c, hipEventRecord(c->ev1, s));
-    HCK(c, hipMemcpyAsync(c->h_out, c->d_out, sizeof(StatRec), hipMemcpyDeviceToHost, s));
-    HCK(c, hipMemcpyAsync(c->h_ctl, c->d_slow_ctl, 16, hipMemcpyDeviceToHost, s));
-    HCK(c, hipStreamSynchronize(s));
-    float f = 0;
-    HCK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
-    if (ms) *ms = f;
-    const uint32_t* ctl = reinterpret_cast<const uint32_t*>(c->h_ctl);
-    if (ctl[2]) return fail(c, GB_ERANGE, "slow-path list overflow (more than 2^20 n with p_min beyond the window)");
-    return GB_OK;
+    return run_segment_on(c, ctx_bufs(c), sp, A, cnt, mp, want_verified, want_pmin, ms);
 }
 
 }  // namespace
@@ -510,6 +599,21 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
     }
     int rc = build_tables(c);
     if (rc) return bail(rc);
+    {
+        // the scratch array's address on this device (current above), looked up once per device
+        std::lock_guard<std::mutex> sk(g_scratch_mu);
+        if (g_scratch.size() <= (size_t)device) g_scratch.resize(device + 1, nullptr);
+        if (!g_scratch[device]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                c->err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+                return bail(GB_EDEVICE);
+            }
+            g_scratch[device] = static_cast<unsigned char*>(p);
+        }
+        c->scratch = g_scratch[device];
+    }
     if (dalloc(c, &c->d_window, window_words) != hipSuccess || dalloc(c, &c->d_verified, verified_words) != hipSuccess ||
         dalloc(c, &c->d_slow_n, SLOW_CAP) != hipSuccess || dalloc(c, &c->d_slow_win, SLOW_CAP) != hipSuccess ||
         dalloc(c, &c->d_slow_res, SLOW_CAP) != hipSuccess || dalloc(c, &c->d_slow_ctl, 2) != hipSuccess ||
@@ -607,6 +711,54 @@ int gb_verify_segment(gb_ctx* c, uint64_t A, uint64_t cnt, uint64_t p_small, uin
     if (verified_out)
         HCK(c, hipMemcpy(verified_out, c->d_verified, ((cnt + 63) / 64) * 8, hipMemcpyDeviceToHost));
     if (pmin_out) HCK(c, hipMemcpy(pmin_out, c->d_pmin, cnt * 4, hipMemcpyDeviceToHost));
+    to_stats(c->h_out[0], ms, st);
+    return GB_OK;
+}
+
+int gb_verify_segment_noalloc(gb_ctx* c, uint64_t A, uint64_t cnt, uint64_t p_small, uint32_t flags,
+                              uint64_t* verified_out, uint32_t* pmin_out, gb_seg_stats* st) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (A < 4 || (A & 1)) return fail(c, GB_EINVAL, "seg_start must be even and >= 4");
+    if (cnt == 0) return fail(c, GB_EINVAL, "seg_even_count must be >= 1");
+    if (cnt - 1 > (~0ull - 1 - A) / 2) return fail(c, GB_EINVAL, "segment runs past 2^64-2");
+    if (cnt > NA_SEG_MAX)
+        return fail(c, GB_ERANGE, "seg_even_count above 2^20, the most gb_verify_segment_noalloc's scratch holds");
+    if (!c->scratch) return fail(c, GB_EDEVICE, "scratch address not set up");
+    HCK(c, hipSetDevice(c->device));
+    unsigned char* base = c->scratch;
+    SegBufs B;
+    B.window = reinterpret_cast<uint64_t*>(base + NA_OFF_WIN);
+    B.window_cap = NA_WIN_WORDS;
+    B.verified = reinterpret_cast<uint64_t*>(base + NA_OFF_VER);
+    B.verified_cap = NA_VER_WORDS;
+    B.pmin = reinterpret_cast<uint32_t*>(base + NA_OFF_PMIN);
+    B.pmin_cap = NA_SEG_MAX;
+    B.rec = reinterpret_cast<StatRec*>(base + NA_OFF_REC);
+    B.rec_cap = NA_RECS;
+    B.r0 = reinterpret_cast<PrimeRes*>(base + NA_OFF_R0);
+    B.r0_cap = SP_TABLE_CAP;
+    B.wins = reinterpret_cast<WinDesc*>(base + NA_OFF_WINS);
+    B.rec_lo = reinterpret_cast<uint64_t*>(base + NA_OFF_LO);
+    B.rec_hi = reinterpret_cast<uint64_t*>(base + NA_OFF_HI);
+    B.out = reinterpret_cast<StatRec*>(base + NA_OFF_OUT);
+    B.slow_ctl = reinterpret_cast<uint64_t*>(base + NA_OFF_SLOW_CTL);
+    B.slow_res = reinterpret_cast<uint64_t*>(base + NA_OFF_SLOW_RES);
+    B.sl.n = reinterpret_cast<uint64_t*>(base + NA_OFF_SLOW_N);
+    B.sl.win = reinterpret_cast<uint32_t*>(base + NA_OFF_SLOW_WIN);
+    B.sl.count = reinterpret_cast<uint32_t*>(B.slow_ctl);
+    B.sl.overflow = reinterpret_cast<uint32_t*>(B.slow_ctl) + 2;
+    B.sl.cap = NA_SLOW_CAP;
+    B.sl.pad = 0;
+    const MarkParams mp = mark_params(c, p_small, flags | (pmin_out ? GB_WANT_PMIN : 0));
+    const SegPlan sp = plan_segment(c, A, cnt);
+    double ms = 0;
+    int rc = run_segment_on(c, B, sp, A, cnt, mp, verified_out != nullptr, pmin_out != nullptr, &ms);
+    if (rc) return rc;
+    if (verified_out)
+        HCK(c, hipMemcpyAsync(verified_out, B.verified, ((cnt + 63) / 64) * 8, hipMemcpyDeviceToHost, c->stream));
+    if (pmin_out) HCK(c, hipMemcpyAsync(pmin_out, B.pmin, cnt * 4, hipMemcpyDeviceToHost, c->stream));
+    HCK(c, hipStreamSynchronize(c->stream));
     to_stats(c->h_out[0], ms, st);
     return GB_OK;
 }

@@ -97,6 +97,8 @@ struct gb_ctx {
     std::vector<std::pair<std::vector<uint64_t>, gb_tuning>> tune_cache;
     // serving engines running on this context's tables
     std::atomic<int> servers{0};
+    // device address of this device's copy of the allocation-free entry point's scratch array
+    unsigned char* scratch = nullptr;
 };
 
 namespace gbh {

@@ -116,6 +116,19 @@ int gb_verify_segment(gb_ctx *ctx, uint64_t seg_start, uint64_t seg_even_count, 
                       uint32_t flags, uint64_t *verified_bits_out, uint32_t *pmin_out,
                       gb_seg_stats *stats);
 
+/* gb_verify_segment without any allocation: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * outputs and errors as gb_verify_segment, with seg_even_count at most 2^20
+ * (GB_ERANGE beyond).  Its device memory is one static array per device
+ * (9.9 MB, held from the first use of the library on that device to the end
+ * of the process) that every call on the device shares: calls that overlap
+ * on one GPU, from any context, race and must be serialised by the caller.
+ * The call issues no hipMalloc, hipFree or other allocating routine. */
+int gb_verify_segment_noalloc(gb_ctx *ctx, uint64_t seg_start, uint64_t seg_even_count, uint64_t p_small,
+                              uint32_t flags, uint64_t *verified_bits_out, uint32_t *pmin_out,
+                              gb_seg_stats *stats);
+
 /* ---- whole ranges ------------------------------------------------------ */
 
 #define GB_FORM_PLAIN   0u  /* per-segment launches + one host sync per segment */
