@@ -3,28 +3,40 @@
 
 One step = one complete verification pass over all 499,999,999,999 even n in
 [4, 1e12] with P_SMALL = 1e6, in the batched persistent form (one fused
-sieve+mark kernel, two workgroups per CU, over mod-6 wheel tiles of 3 * 2^18
-even n from a device queue, replayed as a
-hipGraph).  The plain per-segment form (2^27 evens per segment, one host sync
-per segment, as the reference's loop) is timed on the same range beside it.
+sieve+mark kernel, two workgroups per CU, over mod-6 wheel tiles of 3 * 2^18 even n
+from a device queue, in launches of at most 2^17 tiles, each replayed as a
+hipGraph).  The plain per-segment form (2^27 evens per segment, as the reference's
+loop) is timed on the same range beside it.
 
 The JSON line carries:
-  roofline      the fused wheel kernel: algorithmic bytes (0.375 B per even n:
-                window bitset written + read, verified bitmap written) over
-                its average launch time, measured with HIP events on the
-                library's stream around each launch; traffic from the
-                committed rocprofv3 counter summary when present.
-  plain         the plain form over [4, 1e12]; beside it [4, 1e11] in 2^27-even
-                segments at P_SMALL = 1e6 and 4e9 (the cost must not grow with
-                P_SMALL) and [4, 1e9] in both forms, each against the reference's
-                published RTX 5090 time; --ref-cpu adds the reference's own
-                cpu_goldbach 1000000000 (one thread) built into oracle/_ref.
+  roofline      the fused wheel kernel.  Its binding resource is VALU issue: the
+                wave-instructions one pass issues (rocprofv3 SQ_INSTS_VALU in the
+                committed profile, used only when that profile was taken with the
+                current kernel sources) over the pass's kernel time (HIP events on
+                the library's stream around each launch), against the v_add_u32
+                issue rate the VALU probe measures in the same run.  Beside it the
+                HBM view: algorithmic bytes (0.375 B per even n: window bitset
+                written + read, verified bitmap written) over the same time, and
+                the measured traffic.  Without a current profile the HBM view is
+                the roofline.
+  step_spread   min / median / max / mean / stdev / CV of the timed passes
+                (the protocol of benchmarks/measure_variance.sh:11-28).
+  valu_probe    issue rates of independent v_add_u32, independent v_mad_u64_u32
+                and a dependent v_add_u32 chain (lib/libgb_probe.so).
+  plain         the plain form over [4, 1e12] with its host overhead; beside it
+                [4, 1e11] in 2^27-even segments at P_SMALL = 1e6 and 4e9 (the cost
+                must not grow with P_SMALL), with the sieve and marking kernels
+                timed apart and priced against the HBM roofline, and [4, 1e9] in
+                both forms, each against the reference's published RTX 5090 time,
+                next to the product's cpu_goldbach and the reference's own
+                cpu_goldbach 1000000000 (one thread, built into oracle/_ref; skip
+                it with --no-ref-cpu).
   cpu_baseline  the CPU restatement (oracle/, OpenMP over segments, the
                 reference algorithm: per-n ascending prime scan) timed on this
                 host over 64 sampled 2^24-even segments spread over [4, 1e12].
   high_windows  64 windows above 1e15 (q by pre-sieve + Baillie-PSW) in one
-                persistent launch, evens/s, with Montgomery products/s against
-                the VALU issue ceiling; its CPU baseline is the oracle's
+                persistent launch, evens/s, with its counted VALU issue against
+                the probe's ceiling; its CPU baseline is the oracle's
                 Miller-Rabin restatement on the first 2^19 evens of each window.
   partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
                 candidate pairs/s (the reference loop's p = 2..n/2 per n), with
@@ -42,9 +54,11 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import hashlib
 import json
 import math
 import os
+import statistics
 import subprocess
 import sys
 import time
@@ -69,6 +83,53 @@ REF_CPU_GOLDBACH = os.path.join(ROOT, "oracle", "_ref", "cpu_goldbach")   # the 
 REF_GPU_SECONDS = {10**9: 0.141018, 10**11: 3.3115}   # RESULTS.md:45,47, 1x RTX 5090
 PRESIEVE = 1 << 22             # the high-window tiles pre-sieve with every table prime below 2^22
 SQR_VALU = 28                  # VALU instructions of one 64-bit Montgomery square (mont_sqr, gfx950)
+PRODUCT_CPU_GOLDBACH = os.path.join(ROOT, "goldbach-gpu_amd", "bin", "cpu_goldbach")
+PROBE_LIB = os.path.join(ROOT, "goldbach-gpu_amd", "lib", "libgb_probe.so")
+# sources whose change makes a committed counter profile stale: the kernels and their launch geometry
+PROFILED_SOURCES = [os.path.join(ROOT, "goldbach-gpu_amd", "csrc", f)
+                    for f in ("gb_kernels.hip", "gb_device.hpp", "gb_kernels.hpp", "gb_api.cpp", "gb_ctx.hpp")]
+
+
+def build_hash() -> str:
+    """16 hex digits of SHA-256 over the profiled sources; committed profiles carry the value
+    they were taken with, and a figure read from a profile of other sources is reported as null."""
+    h = hashlib.sha256()
+    for f in PROFILED_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def load_profile(path: str):
+    """A committed counter profile if it was taken with the current sources, else None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        prof = json.load(f)
+    return prof if prof.get("build_hash") == build_hash() else None
+
+
+def valu_probe(kinds=(0, 1, 2), blocks: int = 256 * 64, iters: int = 2048) -> dict:
+    """Achievable VALU issue, measured: gb_probe_valu runs fixed counts of v_add_u32 (kind 0,
+    eight independent chains), v_mad_u64_u32 (kind 1) and one dependent v_add_u32 chain (kind 2)
+    over 16,384 workgroups; rate = wave-instructions issued / launch time (HIP events)."""
+    lib = ctypes.CDLL(PROBE_LIB)
+    lib.gb_probe_valu.restype = ctypes.c_int
+    lib.gb_probe_valu.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    names = {0: "v_add_u32_independent", 1: "v_mad_u64_u32_independent", 2: "v_add_u32_dependent"}
+    out = {}
+    for k in kinds:
+        best = None
+        for _ in range(3):                                  # the first launch loads the code
+            ms, insts = ctypes.c_double(), ctypes.c_double()
+            rc = lib.gb_probe_valu(k, blocks, iters, ctypes.byref(ms), ctypes.byref(insts))
+            if rc != 0:
+                raise RuntimeError(f"gb_probe_valu({k}) failed: {rc}")
+            rate = insts.value / (ms.value * 1e-3)
+            best = rate if best is None else max(best, rate)
+        out[names[k]] = {"wave_insts_per_s": best, "per_cu_clock_at_2.4GHz": best / (CUS * CLOCK_HZ)}
+    return out
 
 
 def device_sync() -> None:
@@ -79,13 +140,41 @@ def device_sync() -> None:
 
 
 def timed_passes(fn, steps: int, warmup: int):
+    """(total seconds, outputs, per-step seconds) of `steps` passes after `warmup` untimed ones."""
     for _ in range(warmup):
         fn()
     device_sync()
     t0 = time.perf_counter()
-    outs = [fn() for _ in range(steps)]
+    outs, per = [], []
+    for _ in range(steps):
+        t = time.perf_counter()
+        outs.append(fn())          # each pass returns after its last launch finished
+        per.append(time.perf_counter() - t)
     device_sync()
-    return time.perf_counter() - t0, outs
+    return time.perf_counter() - t0, outs, per
+
+
+def spread(per_s) -> dict:
+    """Run-to-run spread of the timed passes (benchmarks/measure_variance.sh:11-28 reports
+    mean, standard deviation and coefficient of variation over 20 runs after a warm-up)."""
+    ms = [x * 1e3 for x in per_s]
+    mean = statistics.fmean(ms)
+    sd = statistics.stdev(ms) if len(ms) > 1 else 0.0
+    return {"runs": len(ms), "min_ms": min(ms), "median_ms": statistics.median(ms), "max_ms": max(ms),
+            "mean_ms": mean, "stdev_ms": sd, "cv": sd / mean if mean else None}
+
+
+def host_cpu() -> dict:
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
 
 
 def cpu_baseline() -> dict:
@@ -233,7 +322,11 @@ def high_windows(count: int = 64, lo_evens: int = 10**6, hi_evens: int = 10**9, 
 def plain_1e11(ctx, gb) -> dict:
     """The plain form on [4, 1e11] in 2^27-even segments (the reference's published setting,
     goldbach 100000000000 --seg-size=134217728) at P_SMALL = 1e6 and at the allowed maximum 4e9:
-    the answers agree and the cost does not grow with P_SMALL."""
+    the answers agree and the cost does not grow with P_SMALL.  A third pass on a context with
+    per-kernel timing splits the device time between the sieve and the marking kernels and
+    prices each against the HBM roofline with its algorithmic bytes: the sieve writes one bit
+    per odd q of its window and reads a 16-byte entry per sieving prime; the marking reads the
+    window bits and writes one verified bit per even n."""
     limit, evens = 10**11, (10**11 - 4) // 2 + 1
     out = {"workload": "all even n in [4, 1e11], plain form, 2^27 evens per segment",
            "reference_gpu_seconds": REF_GPU_SECONDS[limit]}
@@ -250,13 +343,46 @@ def plain_1e11(ctx, gb) -> dict:
         assert all(st[k] == ref[k] for k in ("evens", "phase2", "failures"))
         out[f"p_small_{ps:.0e}".replace("+", "")] = {"seconds": dt, "evens_per_s": evens / dt,
                                                      "kernel_ms": st["kernel_ms"],
+                                                     "host_overhead": dt * 1e3 / st["kernel_ms"] - 1.0,
                                                      "vs_reference": REF_GPU_SECONDS[limit] / dt}
+    with gb.Context(0, max_seg_evens=PLAIN_SEG, kernel_times=True) as kc:
+        kc.verify_range(4, 10**9, P_SMALL, form=gb.GB_FORM_PLAIN, seg_evens=PLAIN_SEG)   # warm
+        kc.kernel_times(reset=True)
+        st = kc.verify_range(4, limit, P_SMALL, form=gb.GB_FORM_PLAIN, seg_evens=PLAIN_SEG)
+        kt = kc.kernel_times(reset=True)
+    assert st["evens"] == evens and kt["evens"] == evens
+    sieve_bytes = kt["window_bits"] / 8 + kt["sieve_primes"] * 16
+    mark_bytes = kt["window_bits"] / 8 + kt["evens"] / 8
+    out["kernels"] = {
+        "segments": kt["segments"],
+        "sieve_window_kernel": {"ms": kt["sieve_ms"], "alg_bytes": sieve_bytes,
+                                "achieved_gbs": sieve_bytes / (kt["sieve_ms"] * 1e-3) / 1e9,
+                                "hbm_frac": sieve_bytes / (kt["sieve_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                "odds_per_s": kt["window_bits"] / (kt["sieve_ms"] * 1e-3)},
+        "mark_segment_kernel": {"ms": kt["mark_ms"], "alg_bytes": mark_bytes,
+                                "achieved_gbs": mark_bytes / (kt["mark_ms"] * 1e-3) / 1e9,
+                                "hbm_frac": mark_bytes / (kt["mark_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "other_ms": kt["other_ms"],
+        "note": "HIP events between the launches of every segment (a context opened with "
+                "GB_CFG_KERNEL_TIMES); peak 8 TB/s"}
     return out
 
 
+def cpu_goldbach_run(path: str, limit: int) -> dict:
+    t0 = time.perf_counter()
+    r = subprocess.run([path, str(limit)], capture_output=True, text=True, timeout=900)
+    dt = time.perf_counter() - t0
+    assert r.returncode == 0 and "satisfy Goldbach" in r.stdout, r.stdout + r.stderr
+    evens = (limit - 4) // 2 + 1
+    assert f"Even numbers checked : {evens}" in r.stdout
+    return {"seconds": dt, "evens_per_s": evens / dt}
+
+
 def limit_1e9(ctx, gb, ref_cpu: bool) -> dict:
-    """All even n in [4, 1e9] in the plain and batched forms; with ref_cpu, also the reference's
-    own cpu_goldbach 1000000000 (one thread, built from its sources into oracle/_ref)."""
+    """All even n in [4, 1e9] in the plain and batched forms; beside them the CPU programs over
+    the same range: the product's cpu_goldbach (OpenMP, on the product's segmented_sieve) and,
+    with ref_cpu, the reference's own cpu_goldbach 1000000000 (one thread, built from its
+    sources into oracle/_ref), the CPU baseline of this workload."""
     limit, evens = 10**9, (10**9 - 4) // 2 + 1
     out = {"workload": "all even n in [4, 1e9]", "reference_gpu_seconds": REF_GPU_SECONDS[limit]}
     for name, kw in (("plain", {"form": gb.GB_FORM_PLAIN, "seg_evens": PLAIN_SEG}),
@@ -269,14 +395,14 @@ def limit_1e9(ctx, gb, ref_cpu: bool) -> dict:
         dt = time.perf_counter() - t0
         assert st["evens"] == evens and st["failures"] == 0 and st["phase2"] == 0
         out[name] = {"seconds": dt, "evens_per_s": evens / dt, "vs_reference": REF_GPU_SECONDS[limit] / dt}
+    cpu = host_cpu()
+    if os.path.exists(PRODUCT_CPU_GOLDBACH):
+        out["product_cpu"] = {**cpu_goldbach_run(PRODUCT_CPU_GOLDBACH, limit), "threads": cpu["affinity"],
+                              "program": "goldbach-gpu_amd/bin/cpu_goldbach 1000000000", "host": cpu}
     if ref_cpu and os.path.exists(REF_CPU_GOLDBACH):
-        t0 = time.perf_counter()
-        r = subprocess.run([REF_CPU_GOLDBACH, str(limit)], capture_output=True, text=True, timeout=900)
-        dt = time.perf_counter() - t0
-        assert r.returncode == 0 and "satisfy Goldbach" in r.stdout, r.stdout + r.stderr
-        out["reference_cpu"] = {"seconds": dt, "evens_per_s": evens / dt, "threads": 1,
-                                "program": "oracle/_ref/cpu_goldbach 1000000000 (cpu_goldbach.cpp + "
-                                           "segmented_sieve.cpp, g++ -O3)"}
+        out["reference_cpu"] = {**cpu_goldbach_run(REF_CPU_GOLDBACH, limit), "threads": 1,
+                                "program": "oracle/_ref/cpu_goldbach 1000000000 (the reference's "
+                                           "cpu_goldbach.cpp + segmented_sieve.cpp, g++ -O3)", "host": cpu}
     return out
 
 
@@ -309,8 +435,11 @@ def montgomery_products(wins) -> float:
     return total
 
 
-def windows_leg(ctx) -> dict:
-    """High windows: q decided by pre-sieve + Baillie-PSW, all windows in one persistent launch."""
+def windows_leg(ctx, probe) -> dict:
+    """High windows: q decided by pre-sieve + Baillie-PSW, all windows in one persistent launch.
+    Its roofline is VALU issue: the VALU wave-instructions the launch issues (rocprofv3 count
+    in the committed profile, when that profile was taken with the current sources) over the
+    live launch time, against the v_add_u32 issue rate the probe measures."""
     wins = high_windows()
     evens = sum(e for _, e in wins)
     ctx.verify_windows([(a, min(e, 1 << 20)) for a, e in wins], P_SMALL)   # warm (tables, code)
@@ -322,20 +451,20 @@ def windows_leg(ctx) -> dict:
     assert tot["evens"] == evens and tot["failures"] == 0
     products = montgomery_products(wins)
     ksec = tot["kernel_ms"] * 1e-3
-    valu = None
-    if os.path.exists(WINDOWS_PROFILE):
-        with open(WINDOWS_PROFILE) as f:
-            prof = json.load(f)
-        if prof.get("sq_insts_valu_per_launch"):
-            valu = prof["sq_insts_valu_per_launch"] / (CUS * ksec * CLOCK_HZ)
+    prof = load_profile(WINDOWS_PROFILE)
+    peak = probe["v_add_u32_independent"]["wave_insts_per_s"] if probe else None
+    achieved = prof["sq_insts_valu_per_launch"] / ksec if prof else None
     return {"workload": "64 windows, uniform 1e6..1e9 evens, starts log-uniform in [1e15, 1.8e19] (seed 0)",
             "evens": evens, "value": evens / dt, "unit": "evens/s", "seconds": dt,
             "kernel_ms": tot["kernel_ms"], "phase2": tot["phase2"],
-            "roofline": {"bound": "valu", "unit": "Montgomery products/s", "achieved": products / ksec,
-                         "products": products,
-                         "peak": CUS * CLOCK_HZ * 64 / SQR_VALU, "frac": products / ksec / (CUS * CLOCK_HZ * 64 / SQR_VALU),
-                         "peak_note": "every VALU slot of 256 CUs at 2.4 GHz spent on 28-instruction squarings",
-                         "valu_issue_frac": valu}}
+            "roofline": {"bound": "valu", "unit": "wave-instructions/s", "achieved": achieved, "peak": peak,
+                         "frac": achieved / peak if (achieved and peak) else None,
+                         "kernel": "fused_kernel<1024>",
+                         "counted": "SQ_INSTS_VALU of the launch, profiles/high_windows_mr.json (null when that "
+                                    "profile was taken with other sources)",
+                         "peak_note": "v_add_u32 issue rate of gb_probe_valu, measured in this run",
+                         "montgomery_products_model": products,
+                         "montgomery_products_per_s": products / ksec}}
 
 
 COUNT_LIMIT = 3 * 10**7
@@ -433,11 +562,16 @@ def main() -> int:
     ap.add_argument("--no-windows", action="store_true", help="skip the high-window (Baillie-PSW) leg")
     ap.add_argument("--no-counts", action="store_true", help="skip the partition-count leg")
     ap.add_argument("--no-big", action="store_true", help="skip the big_check (10^1000, 10^10000) leg")
-    ap.add_argument("--ref-cpu", action="store_true",
-                    help="also time the reference's own cpu_goldbach 1000000000 (one thread, 10-60 s)")
+    ap.add_argument("--no-ref-cpu", action="store_true",
+                    help="skip the reference's own cpu_goldbach 1000000000 (one thread, 10-60 s)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the VALU issue probes (for rocprofv3 SQ_INSTS_VALU / SQ_BUSY_CYCLES)")
     args = ap.parse_args()
+    if args.valu_probe:
+        print(json.dumps({"valu_probe": valu_probe(), "build_hash": build_hash()}))
+        return 0
     if args.copy_probe:
         import torch
 
@@ -464,20 +598,30 @@ def main() -> int:
         assert st["evens"] == evens and st["failures"] == 0 and st["phase2"] == 0
         return st
 
-    dt, outs = timed_passes(batched, args.steps, args.warmup)
+    dt, outs, per_step = timed_passes(batched, args.steps, args.warmup)
     value = evens * args.steps / dt
-    kms = float(np.mean([o["kernel_ms"] for o in outs]))
-    achieved = ALG_BYTES_PER_EVEN * evens / (kms * 1e-3) / 1e9
+    kms = float(np.mean([o["kernel_ms"] for o in outs]))           # device time of one pass's launches
+    launches = len(ctx.range_plan(4, LIMIT, P_SMALL))
+    achieved_gbs = ALG_BYTES_PER_EVEN * evens / (kms * 1e-3) / 1e9
 
-    traffic = None
-    valu = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            prof = json.load(f)
-        traffic = prof.get("hbm_bytes_per_launch")
-        if prof.get("sq_insts_valu_per_launch"):
-            # the kernel's real bound: VALU issue (wave-instructions per CU clock, peak 1)
-            valu = prof["sq_insts_valu_per_launch"] / (CUS * kms * 1e-3 * CLOCK_HZ)
+    probe = valu_probe()
+    peak_valu = probe["v_add_u32_independent"]["wave_insts_per_s"]
+    prof = load_profile(TRAFFIC_FILE)         # counters of the current sources, or None
+    traffic = prof.get("hbm_bytes_per_pass") if prof else None
+    hbm = {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+           "alg_bytes_per_pass": ALG_BYTES_PER_EVEN * evens, "traffic_bytes_per_pass": traffic}
+    if prof and prof.get("sq_insts_valu_per_pass"):
+        # the binding resource: VALU issue (counted wave-instructions over the live time)
+        achieved_valu = prof["sq_insts_valu_per_pass"] / (kms * 1e-3)
+        roofline = {"bound": "valu", "achieved": achieved_valu, "peak": peak_valu, "unit": "wave-instructions/s",
+                    "frac": achieved_valu / peak_valu, "traffic": traffic, "hbm": hbm}
+    else:
+        roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                    "valu": None, "valu_note": "no counter profile of the current sources"}
+    roofline.update({"kernel": "wheel_kernel<512, 262144>", "launches_per_pass": launches, "kernel_ms_per_pass": kms,
+                     "kernel_ms_per_launch": kms / launches, "build_hash": build_hash(),
+                     "profile": os.path.relpath(TRAFFIC_FILE, ROOT)})
 
     plain = None
     if not args.no_plain:
@@ -486,13 +630,14 @@ def main() -> int:
             assert st["evens"] == evens and st["failures"] == 0
             return st
         psteps = max(1, min(args.steps, 2))
-        pdt, pouts = timed_passes(plain_pass, psteps, 0)
+        pdt, pouts, _ = timed_passes(plain_pass, psteps, 0)
+        pk = float(np.mean([o["kernel_ms"] for o in pouts]))
         plain = {"value": evens * psteps / pdt, "ms_per_step": pdt * 1e3 / psteps,
-                 "kernel_ms": float(np.mean([o["kernel_ms"] for o in pouts])), "seg_evens": PLAIN_SEG,
-                 "limit_1e11": plain_1e11(ctx, gb), "limit_1e9": limit_1e9(ctx, gb, args.ref_cpu)}
+                 "kernel_ms": pk, "host_overhead": pdt * 1e3 / psteps / pk - 1.0, "seg_evens": PLAIN_SEG,
+                 "limit_1e11": plain_1e11(ctx, gb), "limit_1e9": limit_1e9(ctx, gb, not args.no_ref_cpu)}
 
     serve = None if args.no_serve else serving(ctx)
-    high = None if args.no_windows else windows_leg(ctx)
+    high = None if args.no_windows else windows_leg(ctx, probe)
     counts = None if args.no_counts else counts_leg(ctx)
     big = None if args.no_big else big_leg(ctx)
     ctx.close()
@@ -523,13 +668,9 @@ def main() -> int:
                    "p_small": P_SMALL, "tile_evens": 3 << 18, "workgroups_per_cu": 2,
                    "parallelism": "1 GPU",
                    "plain_seg_evens": PLAIN_SEG},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "wheel_kernel<512, 262144>", "kernel_ms": kms,
-                     "alg_bytes_per_launch": ALG_BYTES_PER_EVEN * evens,
-                     "valu_issue_frac": valu,
-                     "valu_note": "SQ_INSTS_VALU per launch (rocprofv3 summary in profiles/) over 256 CUs x "
-                                  "the live kernel time x 2.4 GHz: the kernel is VALU-issue bound, not HBM bound"},
+        "roofline": roofline,
+        "step_spread": spread(per_step),
+        "valu_probe": probe,
         "cpu_baseline": cpu,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,

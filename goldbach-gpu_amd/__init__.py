@@ -37,8 +37,9 @@ EXPORTS = (
     "gb_min_partition_batch", "gb_autotune", "gb_count_partitions", "gb_verify_range_logged", "gb_log_read",
     "gb_big_min_partition", "gb_big_from_decimal", "gb_big_to_decimal",
     "gb_serve_open", "gb_serve_submit", "gb_serve_poll", "gb_serve_stats_get", "gb_serve_close",
-    "gb_serve_last_error", "gb_range_plan", "gb_verify_segment_noalloc",
+    "gb_serve_last_error", "gb_range_plan", "gb_verify_segment_noalloc", "gb_kernel_times_get",
 )
+GB_CFG_KERNEL_TIMES = 0x1
 GB_KERNEL_PLAIN, GB_KERNEL_WHEEL, GB_KERNEL_FUSED = 0, 1, 2
 
 u64 = ctypes.c_uint64
@@ -47,7 +48,15 @@ u32 = ctypes.c_uint32
 
 class GbConfig(ctypes.Structure):
     _fields_ = [("max_n", u64), ("max_seg_evens", u64), ("phase2_limit", u64), ("mem_budget", u64),
-                ("serve_slots", u32), ("reserved", u32)]
+                ("serve_slots", u32), ("flags", u32)]
+
+
+class GbKernelTimes(ctypes.Structure):
+    _fields_ = [("segments", u64), ("window_bits", u64), ("sieve_primes", u64), ("evens", u64),
+                ("sieve_ms", ctypes.c_double), ("mark_ms", ctypes.c_double), ("other_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class GbSegStats(ctypes.Structure):
@@ -168,6 +177,7 @@ def library() -> ctypes.CDLL:
                                        u64, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
         "gb_autotune": (ctypes.c_int, [vp, u64, u64, u64, u32, u32, ctypes.POINTER(GbTuning),
                                        ctypes.POINTER(ctypes.c_double)]),
+        "gb_kernel_times_get": (ctypes.c_int, [vp, ctypes.POINTER(GbKernelTimes), ctypes.c_int]),
         "gb_range_plan": (ctypes.c_int, [vp, u64, u64, u64, ctypes.POINTER(GbTuning), ctypes.POINTER(GbChunk), u64,
                                          ctypes.POINTER(u64)]),
         "gb_serve_open": (ctypes.c_int, [vp, ctypes.POINTER(GbServeConfig), ctypes.POINTER(vp)]),
@@ -250,9 +260,10 @@ class Context:
     """One device context: resident prime tables, stream and segment buffers."""
 
     def __init__(self, device: int = 0, max_seg_evens: int = 0, phase2_limit: int = 0, max_n: int = 0,
-                 mem_budget: int = 0):
+                 mem_budget: int = 0, kernel_times: bool = False):
         self._lib = library()
-        cfg = GbConfig(max_n, max_seg_evens, phase2_limit, mem_budget, 0, 0)
+        cfg = GbConfig(max_n, max_seg_evens, phase2_limit, mem_budget, 0,
+                       GB_CFG_KERNEL_TIMES if kernel_times else 0)
         h = ctypes.c_void_p()
         rc = self._lib.gb_open(device, ctypes.byref(cfg), ctypes.byref(h))
         if rc != GB_OK:
@@ -283,6 +294,12 @@ class Context:
     def _check(self, rc: int) -> None:
         if rc != GB_OK:
             raise GoldbachError(rc, self._lib.gb_last_error(self._h).decode())
+
+    def kernel_times(self, reset: bool = True) -> dict:
+        """Per-kernel device time of the plain form since the last reset (kernel_times=True contexts)."""
+        kt = GbKernelTimes()
+        self._check(self._lib.gb_kernel_times_get(self._h, ctypes.byref(kt), 1 if reset else 0))
+        return kt.as_dict()
 
     # -- sieve -----------------------------------------------------------
     def sieve_window(self, q_low: int, q_high: int) -> np.ndarray:

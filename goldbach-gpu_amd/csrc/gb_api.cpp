@@ -439,7 +439,7 @@ struct SegPlan {
     uint32_t np;
 };
 
-SegPlan plan_segment(gb_ctx* c, uint64_t A, uint64_t cnt) {
+SegPlan plan_segment(gb_ctx* c, uint64_t A, uint64_t cnt, WinDesc* desc) {
     SegPlan sp;
     sp.g = window_geom(A, cnt + HALO + WIN_PAD);
     sp.wwords = ((uint64_t)sp.g.nbits + 63) / 64;
@@ -447,16 +447,17 @@ SegPlan plan_segment(gb_ctx* c, uint64_t A, uint64_t cnt) {
     sp.nblocks = (sp.vwords + MARK_WPB - 1) / MARK_WPB;
     WinDesc wd{A, cnt, 0, 0, 0, 0, 0, 0, 0};
     set_residues(c, &wd, sp.g.qv0, geom_q_top(sp.g), 0);
-    c->h_wins[0] = wd;
+    *desc = wd;
     sp.np = wd.np;
     return sp;
 }
 
-// One plain segment on buffers that are already large enough; allocates nothing.  Results are
-// left in c->h_out[0].
-int run_segment_on(gb_ctx* c, const SegBufs& B, const SegPlan& sp, uint64_t A, uint64_t cnt, const MarkParams& mp,
-                   bool want_verified, bool want_pmin, double* ms) {
+// Queues one plain segment on buffers that are already large enough, staged through slot
+// `slot` (its descriptor already written by plan_segment); allocates nothing and does not wait.
+int enqueue_segment(gb_ctx* c, const SegBufs& B, const SegPlan& sp, uint32_t slot, uint64_t A, uint64_t cnt,
+                    const MarkParams& mp, bool want_verified, bool want_pmin) {
     hipStream_t s = c->stream;
+    gb_ctx::SegSlot& S = c->seg_slot[slot];
     const ItemTable* it = nullptr;       // the plain tile schedule, built by gb_open
     for (const ItemTable& t : c->items)
         if (t.tb == PLAIN_TILE_BITS && !t.wheel) it = &t;
@@ -465,40 +466,70 @@ int run_segment_on(gb_ctx* c, const SegBufs& B, const SegPlan& sp, uint64_t A, u
     if (sp.wwords > B.window_cap || sp.vwords > B.verified_cap || (want_pmin && cnt > B.pmin_cap) ||
         sp.nblocks > B.rec_cap || sp.np > B.r0_cap)
         return fail(c, GB_ERANGE, "segment larger than the buffers it runs on");
-    c->h_lohi[0] = 0;
-    c->h_lohi[1] = sp.nblocks;
-    HCK(c, hipMemcpyAsync(B.wins, c->h_wins, sizeof(WinDesc), hipMemcpyHostToDevice, s));
-    HCK(c, hipMemcpyAsync(B.rec_lo, &c->h_lohi[0], 8, hipMemcpyHostToDevice, s));
-    HCK(c, hipMemcpyAsync(B.rec_hi, &c->h_lohi[1], 8, hipMemcpyHostToDevice, s));
-    HCK(c, hipEventRecord(c->ev0, s));
-    HCK(c, launch_fill(s, B.slow_ctl, 2, 0));
-    HCK(c, launch_residues(s, T, B.wins, 1, sp.np, B.r0));
+    HCK(c, hipEventRecord(S.ev0, s));
+    const bool timed = (c->cfg.flags & GB_CFG_KERNEL_TIMES) != 0;
+    S.bits = sp.g.nbits;
+    S.primes = sp.np;
+    S.evens = cnt;
+    // descriptor, record bounds, slow-list reset and prime entries in one launch (no copies)
+    HCK(c, launch_segment_prep(s, T, *S.h_win, sp.nblocks, B.wins, B.rec_lo, B.rec_hi, B.slow_ctl, B.r0));
+    if (timed) HCK(c, hipEventRecord(S.ev_sieve, s));
     HCK(c, launch_sieve_window(s, T, sp.g.qv0, sp.g.lead, sp.g.nbits, PLAIN_TILE_BITS, B.window, 0, B.r0));
+    if (timed) HCK(c, hipEventRecord(S.ev_mark, s));
     uint32_t nrec = 0;
     HCK(c, launch_mark_segment(s, T, mp, A, cnt, B.window, sp.wwords, MARK_WPB, want_verified ? B.verified : nullptr,
                                want_pmin ? B.pmin : nullptr, B.sl, B.rec, &nrec));
+    if (timed) HCK(c, hipEventRecord(S.ev_rest, s));
     HCK(c, launch_slow(s, T, mp, B.sl, B.slow_res, B.wins, want_verified ? B.verified : nullptr,
                        want_pmin ? B.pmin : nullptr, A));
-    HCK(c, launch_reduce(s, mp, B.rec, B.rec_lo, B.rec_hi, 1, B.sl, B.slow_res, B.out));
-    HCK(c, hipEventRecord(c->ev1, s));
-    HCK(c, hipMemcpyAsync(c->h_out, B.out, sizeof(StatRec), hipMemcpyDeviceToHost, s));
-    HCK(c, hipMemcpyAsync(c->h_ctl, B.slow_ctl, 16, hipMemcpyDeviceToHost, s));
-    HCK(c, hipStreamSynchronize(s));
+    // the record goes straight to the slot's pinned host memory (no read-back copy)
+    HCK(c, launch_reduce(s, mp, B.rec, B.rec_lo, B.rec_hi, 1, B.sl, B.slow_res, S.h_out));
+    HCK(c, hipEventRecord(S.ev1, s));
+    HCK(c, hipEventRecord(S.done, s));
+    return GB_OK;
+}
+
+// Waits for the segment queued in `slot`: its record to *out, its kernel time to *ms.
+int finish_segment(gb_ctx* c, uint32_t slot, uint32_t slow_cap, StatRec* out, double* ms) {
+    gb_ctx::SegSlot& S = c->seg_slot[slot];
+    HCK(c, hipEventSynchronize(S.done));
     float f = 0;
-    HCK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    HCK(c, hipEventElapsedTime(&f, S.ev0, S.ev1));
     if (ms) *ms = f;
-    const uint32_t* ctl = reinterpret_cast<const uint32_t*>(c->h_ctl);
-    if (ctl[2])
-        return fail(c, GB_ERANGE, "slow-path list overflow (more than " + std::to_string(B.sl.cap) +
+    if (c->cfg.flags & GB_CFG_KERNEL_TIMES) {
+        float fs = 0, fm = 0;
+        HCK(c, hipEventElapsedTime(&fs, S.ev_sieve, S.ev_mark));
+        HCK(c, hipEventElapsedTime(&fm, S.ev_mark, S.ev_rest));
+        gb_kernel_times& K = c->ktimes;
+        K.segments += 1;
+        K.window_bits += S.bits;
+        K.sieve_primes += S.primes;
+        K.evens += S.evens;
+        K.sieve_ms += fs;
+        K.mark_ms += fm;
+        K.other_ms += f - fs - fm;
+    }
+    *out = *S.h_out;
+    if (out->pad)
+        return fail(c, GB_ERANGE, "slow-path list overflow (more than " + std::to_string(slow_cap) +
                                       " n with p_min beyond the window)");
     return GB_OK;
 }
 
-// One plain segment already validated, on the context's buffers (grown as needed).
-int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool want_verified, bool want_pmin,
-                double* ms) {
-    const SegPlan sp = plan_segment(c, A, cnt);
-    if (sp.wwords > c->window_cap || sp.vwords > c->verified_cap)
+// One plain segment on buffers that are already large enough; allocates nothing.
+int run_segment_on(gb_ctx* c, const SegBufs& B, uint64_t A, uint64_t cnt, const MarkParams& mp, bool want_verified,
+                   bool want_pmin, double* ms, StatRec* out) {
+    const SegPlan sp = plan_segment(c, A, cnt, c->seg_slot[0].h_win);
+    int rc = enqueue_segment(c, B, sp, 0, A, cnt, mp, want_verified, want_pmin);
+    if (rc) return rc;
+    return finish_segment(c, 0, B.sl.cap, out, ms);
+}
+
+// The context's buffers grown for plain segments of up to cnt evens (r0 to the whole table, so a
+// range's later segments never reallocate it while earlier ones run).
+int ensure_plain(gb_ctx* c, uint64_t cnt, bool want_pmin) {
+    const uint64_t vwords = (cnt + 63) / 64;
+    if ((cnt + HALO + WIN_PAD + 63) / 64 + 1 > c->window_cap || vwords > c->verified_cap)
         return fail(c, GB_ERANGE, "segment larger than max_seg_evens");
     if (want_pmin && cnt > c->pmin_cap) {
         dfree(c, c->d_pmin);
@@ -507,11 +538,17 @@ int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool 
         HCK(c, dalloc(c, &c->d_pmin, cnt));
         c->pmin_cap = cnt;
     }
-    int rc = ensure_rec(c, sp.nblocks);
+    int rc = ensure_rec(c, (vwords + MARK_WPB - 1) / MARK_WPB);
     if (rc) return rc;
-    rc = ensure_r0(c, sp.np);
+    return ensure_r0(c, c->nsp);
+}
+
+// One plain segment already validated, on the context's buffers (grown as needed).
+int run_segment(gb_ctx* c, uint64_t A, uint64_t cnt, const MarkParams& mp, bool want_verified, bool want_pmin,
+                double* ms, StatRec* out) {
+    int rc = ensure_plain(c, cnt, want_pmin);
     if (rc) return rc;
-    return run_segment_on(c, ctx_bufs(c), sp, A, cnt, mp, want_verified, want_pmin, ms);
+    return run_segment_on(c, ctx_bufs(c), A, cnt, mp, want_verified, want_pmin, ms, out);
 }
 
 }  // namespace
@@ -545,6 +582,15 @@ int gb_device_info(int device, char* name, size_t name_len, uint64_t* total_mem,
     }
     if (total_mem) *total_mem = p.totalGlobalMem;
     if (compute_units) *compute_units = p.multiProcessorCount;
+    return GB_OK;
+}
+
+int gb_kernel_times_get(gb_ctx* c, gb_kernel_times* out, int reset) {
+    if (!c) return GB_EINVAL;
+    c->err.clear();
+    if (!out) return fail(c, GB_EINVAL, "gb_kernel_times_get: out is NULL");
+    *out = c->ktimes;
+    if (reset) c->ktimes = gb_kernel_times{};
     return GB_OK;
 }
 
@@ -627,6 +673,28 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
         c->err = "pinned allocation failed";
         return bail(GB_ENOMEM);
     }
+    {
+        // per-slot staging of the plain segments in flight: 256 pinned bytes per slot
+        constexpr size_t SLOT = 256;
+        static_assert(sizeof(WinDesc) + sizeof(StatRec) <= SLOT, "slot staging fits");
+        if (hipHostMalloc(&c->h_slot_block, 2 * SLOT, hipHostMallocDefault) != hipSuccess) {
+            c->err = "pinned allocation failed";
+            return bail(GB_ENOMEM);
+        }
+        for (int k = 0; k < 2; ++k) {
+            unsigned char* b = static_cast<unsigned char*>(c->h_slot_block) + k * SLOT;
+            gb_ctx::SegSlot& S = c->seg_slot[k];
+            S.h_win = reinterpret_cast<WinDesc*>(b);
+            S.h_out = reinterpret_cast<StatRec*>(b + sizeof(WinDesc));
+            if (hipEventCreate(&S.ev0) != hipSuccess || hipEventCreate(&S.ev1) != hipSuccess ||
+                hipEventCreateWithFlags(&S.done, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreate(&S.ev_sieve) != hipSuccess || hipEventCreate(&S.ev_mark) != hipSuccess ||
+                hipEventCreate(&S.ev_rest) != hipSuccess) {
+                c->err = "hipEventCreate failed";
+                return bail(GB_EDEVICE);
+            }
+        }
+    }
     rc = ensure_wins(c, 64);
     if (rc) return bail(rc);
     *out = c;
@@ -656,6 +724,10 @@ int gb_close(gb_ctx* c) {
     if (c->h_ctl) dhostfree(c->h_ctl);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (auto& S : c->seg_slot)
+        for (hipEvent_t e : {S.ev0, S.ev1, S.done, S.ev_sieve, S.ev_mark, S.ev_rest})
+            if (e) (void)hipEventDestroy(e);
+    if (c->h_slot_block) dhostfree(c->h_slot_block);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GB_OK;
@@ -706,12 +778,13 @@ int gb_verify_segment(gb_ctx* c, uint64_t A, uint64_t cnt, uint64_t p_small, uin
     HCK(c, hipSetDevice(c->device));
     const MarkParams mp = mark_params(c, p_small, flags | (pmin_out ? GB_WANT_PMIN : 0));
     double ms = 0;
-    int rc = run_segment(c, A, cnt, mp, verified_out != nullptr, pmin_out != nullptr, &ms);
+    StatRec rec;
+    int rc = run_segment(c, A, cnt, mp, verified_out != nullptr, pmin_out != nullptr, &ms, &rec);
     if (rc) return rc;
     if (verified_out)
         HCK(c, hipMemcpy(verified_out, c->d_verified, ((cnt + 63) / 64) * 8, hipMemcpyDeviceToHost));
     if (pmin_out) HCK(c, hipMemcpy(pmin_out, c->d_pmin, cnt * 4, hipMemcpyDeviceToHost));
-    to_stats(c->h_out[0], ms, st);
+    to_stats(rec, ms, st);
     return GB_OK;
 }
 
@@ -751,15 +824,15 @@ int gb_verify_segment_noalloc(gb_ctx* c, uint64_t A, uint64_t cnt, uint64_t p_sm
     B.sl.cap = NA_SLOW_CAP;
     B.sl.pad = 0;
     const MarkParams mp = mark_params(c, p_small, flags | (pmin_out ? GB_WANT_PMIN : 0));
-    const SegPlan sp = plan_segment(c, A, cnt);
     double ms = 0;
-    int rc = run_segment_on(c, B, sp, A, cnt, mp, verified_out != nullptr, pmin_out != nullptr, &ms);
+    StatRec rec;
+    int rc = run_segment_on(c, B, A, cnt, mp, verified_out != nullptr, pmin_out != nullptr, &ms, &rec);
     if (rc) return rc;
     if (verified_out)
         HCK(c, hipMemcpyAsync(verified_out, B.verified, ((cnt + 63) / 64) * 8, hipMemcpyDeviceToHost, c->stream));
     if (pmin_out) HCK(c, hipMemcpyAsync(pmin_out, B.pmin, cnt * 4, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
-    to_stats(c->h_out[0], ms, st);
+    to_stats(rec, ms, st);
     return GB_OK;
 }
 
@@ -1035,26 +1108,53 @@ int plan_batched(gb_ctx* c, uint64_t start, uint64_t limit, const gb_tuning& t, 
     return GB_OK;
 }
 
-// Plain form over [start, limit]: one run_segment per seg evens, stopping at the first failure.
+// Plain form over [start, limit]: segments of seg evens, stopping at the first failure.  Two
+// segments are in flight: segment k + 1 is queued before the host waits for segment k, so the
+// device never idles on the host's per-segment read-back (the reference syncs the stream after
+// every segment, goldbach.cu:462-463).
 int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, const MarkParams& mp,
                      gb_seg_stats* tot, volatile uint64_t* progress) {
+    int rc = ensure_plain(c, std::min(seg, (limit - start) / 2 + 1), false);
+    if (rc) return rc;
+    const SegBufs B = ctx_bufs(c);
     uint64_t done = 0;
-    for (uint64_t a = start;;) {
+    uint64_t a = start, queued = 0, finished = 0;
+    uint64_t cnt_of[2] = {0, 0};
+    bool more = true;
+    auto queue_next = [&]() -> int {
         const uint64_t left = (limit - a) / 2 + 1;
         const uint64_t cnt = std::min(left, seg);
+        const uint32_t slot = (uint32_t)(queued & 1);
+        const SegPlan sp = plan_segment(c, a, cnt, c->seg_slot[slot].h_win);
+        // the verified map is written, as the reference does
+        int r = enqueue_segment(c, B, sp, slot, a, cnt, mp, true, false);
+        if (r) return r;
+        cnt_of[slot] = cnt;
+        ++queued;
+        if (cnt == left) more = false;
+        else a += 2 * cnt;
+        return GB_OK;
+    };
+    rc = queue_next();
+    while (rc == GB_OK && finished < queued) {
+        if (more) rc = queue_next();
+        if (rc) break;
+        const uint32_t slot = (uint32_t)(finished & 1);
+        StatRec r;
         double ms = 0;
-        int rc = run_segment(c, a, cnt, mp, true, false, &ms);   // verified map written, as the reference does
-        if (rc) return rc;
+        rc = finish_segment(c, slot, B.sl.cap, &r, &ms);
+        if (rc) break;
+        ++finished;
         gb_seg_stats one{};
-        to_stats(c->h_out[0], ms, &one);
+        to_stats(r, ms, &one);
         add_stats(tot, one);
-        done += cnt;
+        done += cnt_of[slot];
         if (progress) *progress = done;
         if (one.failures) break;        // the reference stops at the first failure (goldbach.cu:420,476-480)
-        if (cnt == left) break;
-        a += 2 * cnt;
     }
-    return GB_OK;
+    // a segment queued past a failure or an error runs out; its results are not counted
+    HCK(c, hipStreamSynchronize(c->stream));
+    return rc;
 }
 
 }  // namespace

@@ -86,6 +86,18 @@ struct gb_ctx {
     uint64_t* h_lohi = nullptr;        // [rec_lo | rec_hi] of those pieces
     uint64_t* h_ctl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // plain segments in flight, two deep: each slot's pinned descriptor and result record (the
+    // reduction writes it there) and events (kernels start, kernels end, segment done)
+    struct SegSlot {
+        gbk::WinDesc* h_win = nullptr;
+        gbk::StatRec* h_out = nullptr;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+        hipEvent_t ev_sieve = nullptr, ev_mark = nullptr, ev_rest = nullptr;   // GB_CFG_KERNEL_TIMES
+        uint64_t bits = 0, primes = 0, evens = 0;                              // of the queued segment
+    };
+    SegSlot seg_slot[2];
+    void* h_slot_block = nullptr;
+    gb_kernel_times ktimes{};
     // batched launches captured as graphs; a graph's kernels read their pieces from d_wins, so
     // one graph serves every chunk of the same shape
     struct Graph {

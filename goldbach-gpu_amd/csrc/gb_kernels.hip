@@ -1295,7 +1295,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(MarkParams mp, const StatRe
         r.max_pmin = t.mx;
         r.argmax = t.arg;
         r.evens = ev;
-        r.pad = 0;
+        r.pad = *sl.overflow;   // the slow list overflowed: these statistics are incomplete
         out[w] = r;
     }
 }
@@ -1305,6 +1305,31 @@ __global__ __launch_bounds__(256) void reduce_kernel(MarkParams mp, const StatRe
 // even, else (-rbase mod p + p) / 2 (rbase odd, so rbase + 2b is the odd multiple).
 __global__ void residue_kernel(Tables T, const WinDesc* __restrict__ wins, PrimeRes* __restrict__ pr) {
     const WinDesc wd = wins[blockIdx.y];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < wd.np; k += gridDim.x * blockDim.x) {
+        const uint32_t p = T.sp[k];
+        const uint32_t r = mod_small(wd.rbase, p, T.inv[k], T.c32[k]);
+        const uint32_t t = r ? p - r : 0;
+        PrimeRes e;
+        e.p = p;
+        e.e = (t & 1) ? (t + p) >> 1 : t >> 1;
+        e.inv = T.inv[k];
+        pr[wd.r0off + k] = e;
+    }
+}
+
+// Set-up of one plain segment in a single launch: its descriptor, record bounds and cleared
+// slow-list control (block 0), and the sieving-prime entries of its window (all blocks), so no
+// copy precedes the segment's kernels.
+__global__ void segment_prep_kernel(Tables T, WinDesc wd, uint64_t nrec, WinDesc* __restrict__ wins,
+                                    uint64_t* __restrict__ rec_lo, uint64_t* __restrict__ rec_hi,
+                                    uint64_t* __restrict__ slow_ctl, PrimeRes* __restrict__ pr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        wins[0] = wd;
+        rec_lo[0] = 0;
+        rec_hi[0] = nrec;
+        slow_ctl[0] = 0;
+        slow_ctl[1] = 0;
+    }
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < wd.np; k += gridDim.x * blockDim.x) {
         const uint32_t p = T.sp[k];
         const uint32_t r = mod_small(wd.rbase, p, T.inv[k], T.c32[k]);
@@ -1471,16 +1496,33 @@ __global__ __launch_bounds__(64 * SERVE_WAVES) void min_partition_kernel(Tables 
     if (lane == 0) p_out[qi] = res;
 }
 
-// Continuous-batching step: every admitted query (one wave each, new and
-// in-flight alike) advances by up to `chunks` chunks from its saved position.
-__global__ __launch_bounds__(64 * SERVE_WAVES) void serve_step_kernel(Tables T, const ServeIn* __restrict__ in,
-                                                                      uint32_t count, uint64_t bound, uint32_t chunks,
-                                                                      ServeOut* __restrict__ out) {
+// Continuous-batching step over slots [0, count): a slot whose adm entry is nonzero takes that
+// n as a new query (scan from the first candidate); otherwise it resumes the query its
+// device-resident state holds (n = 0: the slot is free).  One wave per slot advances its query
+// by up to `chunks` chunks; out[slot] (pinned host memory) gets the answer and whether it is
+// final, and the state keeps the position of a query that goes on (or frees the slot).
+__global__ __launch_bounds__(64 * SERVE_WAVES) void serve_step_kernel(Tables T, const uint64_t* __restrict__ adm,
+                                                                      ServeSlot* __restrict__ slots, uint32_t count,
+                                                                      uint64_t bound, uint32_t chunks,
+                                                                      ServeDone* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t qi = blockIdx.x * SERVE_WAVES + (threadIdx.x >> 6);
     if (qi >= count) return;   // uniform per wave
-    const uint64_t n = in[qi].n;
-    uint64_t base = in[qi].k, res = 0;
+    const uint64_t a = adm[qi];
+    uint64_t n, base;
+    if (a) {
+        n = a;
+        base = 0;
+    } else {
+        const ServeSlot st = slots[qi];
+        n = st.n;
+        base = st.k;
+    }
+    if (n == 0) {
+        if (lane == 0) out[qi] = ServeDone{0, 2, 0};
+        return;
+    }
+    uint64_t res = 0;
     uint32_t state = 1;
     if (n < 4 || (n & 1)) {
         res = 0;
@@ -1496,12 +1538,8 @@ __global__ __launch_bounds__(64 * SERVE_WAVES) void serve_step_kernel(Tables T, 
         }
     }
     if (lane == 0) {
-        ServeOut o;
-        o.p = res;
-        o.k = base;
-        o.state = state;
-        o.pad = 0;
-        out[qi] = o;
+        out[qi] = ServeDone{res, state, 0};
+        slots[qi] = state ? ServeSlot{0, 0} : ServeSlot{n, base};
     }
 }
 
@@ -1636,6 +1674,15 @@ hipError_t launch_reduce(hipStream_t s, const MarkParams& mp, const StatRec* rec
     return hipGetLastError();
 }
 
+hipError_t launch_segment_prep(hipStream_t s, const Tables& T, const WinDesc& wd, uint64_t nrec, WinDesc* wins,
+                               uint64_t* rec_lo, uint64_t* rec_hi, uint64_t* slow_ctl, PrimeRes* pr) {
+    uint32_t bx = (wd.np + 255) / 256;
+    if (bx < 1) bx = 1;
+    if (bx > 1024) bx = 1024;
+    hipLaunchKernelGGL(segment_prep_kernel, dim3(bx), dim3(256), 0, s, T, wd, nrec, wins, rec_lo, rec_hi, slow_ctl, pr);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(hipStream_t s, uint64_t* p, uint64_t words, uint64_t value) {
     if (!words) return hipSuccess;
     uint64_t blocks = (words + 255) / 256;
@@ -1663,12 +1710,12 @@ hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, doubl
     return hipGetLastError();
 }
 
-hipError_t launch_serve_step(hipStream_t s, const Tables& T, const ServeIn* in, uint32_t count, uint64_t bound,
-                             uint32_t chunks, ServeOut* out) {
+hipError_t launch_serve_step(hipStream_t s, const Tables& T, const uint64_t* adm, ServeSlot* slots, uint32_t count,
+                             uint64_t bound, uint32_t chunks, ServeDone* out) {
     if (!count) return hipSuccess;
     const uint32_t blocks = (count + SERVE_WAVES - 1) / SERVE_WAVES;
-    hipLaunchKernelGGL(serve_step_kernel, dim3(blocks), dim3(64 * SERVE_WAVES), 0, s, T, in, count, bound, chunks,
-                       out);
+    hipLaunchKernelGGL(serve_step_kernel, dim3(blocks), dim3(64 * SERVE_WAVES), 0, s, T, adm, slots, count, bound,
+                       chunks, out);
     return hipGetLastError();
 }
 

@@ -119,10 +119,16 @@ hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, co
 hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
                        const WinDesc* wins, uint64_t* verified, uint32_t* pmin_out, uint64_t pmin_n0);
 
-// Merge per-block records of window w in [rec_lo[w], rec_hi[w]) plus its slow entries.
+// Merge per-block records of window w in [rec_lo[w], rec_hi[w]) plus its slow entries into out[w]
+// (device or pinned host memory); out[w].pad is nonzero when the slow list overflowed.
 hipError_t launch_reduce(hipStream_t s, const MarkParams& mp, const StatRec* rec, const uint64_t* rec_lo,
                          const uint64_t* rec_hi, uint32_t nwin, SlowList sl, const uint64_t* res_p,
                          StatRec* out);
+
+// One plain segment's set-up: descriptor wins[0] = wd, record bounds [0, nrec), slow-list
+// control cleared, and pr[k] for k < wd.np (the window's sieving-prime entries).
+hipError_t launch_segment_prep(hipStream_t s, const Tables& T, const WinDesc& wd, uint64_t nrec, WinDesc* wins,
+                               uint64_t* rec_lo, uint64_t* rec_hi, uint64_t* slow_ctl, PrimeRes* pr);
 
 // Zero a set of 64-bit words (graph-safe replacement for hipMemsetAsync).
 hipError_t launch_fill(hipStream_t s, uint64_t* p, uint64_t words, uint64_t value);
@@ -139,20 +145,22 @@ hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, doubl
 hipError_t launch_count_partitions(hipStream_t s, const uint64_t* P, uint64_t* R, uint64_t M, uint64_t pwords,
                                    uint64_t start, uint64_t n_end, uint64_t* counts);
 
-// Serving step records: query n resumes its scan at candidate index k.
-struct ServeIn {
+// Serving: a slot's device-resident state (n = 0: free) and its per-step answer in pinned host
+// memory (state 0: goes on, 1: final answer p, 2: the slot was free).
+struct ServeSlot {
     uint64_t n;
-    uint64_t k;
+    uint64_t k;       // next candidate index of the scan
 };
-struct ServeOut {
+struct ServeDone {
     uint64_t p;       // minimal p (state 1), 0 when none up to the bound
-    uint64_t k;       // next candidate index (state 0)
-    uint32_t state;   // 1 done, 0 continue
+    uint32_t state;
     uint32_t pad;
 };
 
-hipError_t launch_serve_step(hipStream_t s, const Tables& T, const ServeIn* in, uint32_t count, uint64_t bound,
-                             uint32_t chunks, ServeOut* out);
+// One step over slots [0, count): adm[i] != 0 starts query adm[i] in slot i (adm and out are
+// pinned host memory, slots device memory).
+hipError_t launch_serve_step(hipStream_t s, const Tables& T, const uint64_t* adm, ServeSlot* slots, uint32_t count,
+                             uint64_t bound, uint32_t chunks, ServeDone* out);
 
 // Serving: p_out[i] = minimal p for ns[i].
 hipError_t launch_min_partition(hipStream_t s, const Tables& T, const uint64_t* ns, uint32_t count, uint64_t bound,

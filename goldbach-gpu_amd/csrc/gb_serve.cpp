@@ -1,18 +1,21 @@
 // gb_serve.cpp -- the continuous-batching serving engine of goldbach_gpu.h.
 //
-// One engine thread per server owns a HIP stream and a slot table in pinned host
-// memory that the step kernel reads and writes directly (a few bytes per query,
-// so no copy commands sit between a launch and its answers).
-// Each iteration (a "step") it
-//   1. moves queued queries into free slots (at most `slots` resident: the
-//      slot count is what the memory budget allows),
-//   2. launches serve_step_kernel over every resident query, new and
-//      in-flight alike (one wave per query, up to chunks_per_step chunks of
-//      64 candidate primes from the query's saved position),
-//   3. retires the queries that finished and keeps the rest resident with
-//      their new position.
-// Submitters and pollers only touch host queues under a mutex, so arrivals
-// overlap with the step running on the device and join at the next step.
+// A server's slots form two lanes, each with its own engine thread and HIP stream.  A slot's query
+// state (n and scan position, 16 bytes) lives in device memory, allocated from the
+// context's budget, so the number of resident queries is what the device memory granted
+// to the server holds.  Per slot the host keeps two pinned words the step kernel reads
+// and writes directly: the n of a query admitted this step, and the step's answer.
+// Each iteration (a "step") a lane's engine
+//   1. moves queued queries into its free slots (the lowest free ones first),
+//   2. launches serve_step_kernel over its occupied range, new and in-flight queries
+//      alike (one wave per slot, up to chunks_per_step chunks of 64 candidate primes
+//      from the query's saved position, which stays on the device),
+//   3. retires the queries that finished; the others keep their slot.
+// The two lanes' host work runs on two cores and their steps overlap on the device.
+// Submitters and pollers only touch host queues under a mutex, so arrivals overlap
+// with the step running on the device and join at the next step.  Queues are vectors
+// appended and drained in bulk, and the engine builds a step's results without the
+// lock, so a burst costs a few nanoseconds of host work per query.
 // An idle engine polls an arrival counter for up to 200 us before it sleeps on
 // its condition variable, so a query arriving at an idle server does not wait for
 // the thread to be woken (the price: one host core busy while queries keep coming).
@@ -26,7 +29,7 @@
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
-#include <deque>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -42,7 +45,7 @@ namespace {
 constexpr uint32_t DEFAULT_CHUNKS = 4;
 constexpr uint64_t DEFAULT_QUEUE = 1ull << 22;
 constexpr uint64_t IDLE_SPIN_NS = 200000;   // an idle engine polls this long for arrivals before it sleeps
-constexpr uint64_t SLOT_BYTES = sizeof(ServeIn) + sizeof(ServeOut);   // slot state (pinned, device-visible)
+constexpr uint64_t SLOT_DEVICE_BYTES = sizeof(ServeSlot);   // device memory one resident query holds
 
 uint64_t now_ns() {
     timespec ts;
@@ -53,9 +56,70 @@ uint64_t now_ns() {
 struct Query {
     uint64_t id;
     uint64_t n;
-    uint64_t k;          // next candidate index of the scan
+    uint64_t submit_ns;
+};
+
+struct SlotInfo {   // host side of an occupied slot (n = 0: free)
+    uint64_t id;
+    uint64_t n;
     uint64_t submit_ns;
     uint32_t steps;
+};
+
+// A FIFO of trivially copyable records appended and drained in bulk.
+template <typename T>
+struct Fifo {
+    std::vector<T> v;
+    size_t head = 0;
+    size_t size() const { return v.size() - head; }
+    bool empty() const { return head == v.size(); }
+    // Drops what was drained: for free once all of it was, otherwise only when the part still
+    // queued is a sixteenth of the part drained, so the move it costs stays amortised.
+    void compact() {
+        if (head == v.size()) {
+            v.clear();
+            head = 0;
+        } else if (head > 65536 && v.size() - head < head / 16) {
+            v.erase(v.begin(), v.begin() + (ptrdiff_t)head);
+            head = 0;
+        }
+    }
+};
+
+// Free slots as a bitmap; take() hands out the lowest, so the occupied slots stay packed at the
+// bottom of the table and a step launches over no more slots than it must.
+struct FreeSlots {
+    std::vector<uint64_t> bits;
+    size_t hint = 0;   // no free slot below word `hint`
+    uint32_t count;
+    explicit FreeSlots(uint32_t n) : bits((n + 63) / 64, ~0ull), count(n) {
+        if (n % 64) bits.back() = (1ull << (n % 64)) - 1;
+    }
+    size_t size() const { return count; }
+    uint32_t take() {   // callers check size() first
+        while (!bits[hint]) ++hint;
+        const uint32_t b = (uint32_t)__builtin_ctzll(bits[hint]);
+        bits[hint] &= bits[hint] - 1;
+        --count;
+        return (uint32_t)(hint * 64 + b);
+    }
+    void give(uint32_t i) {
+        bits[i / 64] |= 1ull << (i % 64);
+        hint = std::min<size_t>(hint, i / 64);
+        ++count;
+    }
+};
+
+// A lane: half of the server's slots with its own engine thread and stream.  Its query
+// state is in device memory; per slot two pinned words the step kernel reads and writes:
+// the n of a query admitted this step, and the step's answer.
+struct Lane {
+    uint32_t size = 0;              // slots
+    ServeSlot* d_slots = nullptr;   // within the server's device table
+    uint64_t* adm = nullptr;
+    ServeDone* done = nullptr;
+    hipStream_t stream = nullptr;
+    std::thread engine;
 };
 
 }  // namespace
@@ -68,14 +132,18 @@ struct gb_server {
     uint32_t slots = 0;
     uint32_t chunks = 0;
     uint64_t queue_cap = 0;
-    hipStream_t stream = nullptr;
-    ServeIn* h_in = nullptr;    // slot state in pinned host memory, read and written by the step kernel
-    ServeOut* h_out = nullptr;
+    ServeSlot* d_slots = nullptr;   // query state, device memory (counted against the context's budget)
+    Lane lanes[2];                  // two engines, each on half of the slots
+    uint64_t lane_resident[2] = {0, 0};
+    std::mutex submit_mu;           // one submission at a time, so its ids stay consecutive
     std::mutex mu;
-    std::condition_variable cv_work;   // engine waits for queries or stop
+    std::condition_variable cv_work;   // lane 0 waits for queries or stop
+    std::condition_variable cv_help;   // lane 1 waits for what lane 0 leaves queued
     std::condition_variable cv_done;   // pollers wait for results
-    std::deque<Query> queue;
-    std::deque<gb_result> done;
+    Fifo<Query> queue;
+    Fifo<gb_result> done;              // finished, not yet taken by a poller
+    std::mutex poll_mu;                // pollers, one at a time
+    Fifo<gb_result> polled;            // taken by pollers, not yet returned
     uint64_t next_id = 0;
     std::atomic<uint64_t> arrivals{0};   // submissions so far: read without the lock by an idle engine
     std::atomic<bool> stopping{false};
@@ -83,7 +151,6 @@ struct gb_server {
     bool failed = false;
     std::string err;
     gb_serve_stats st{};
-    std::thread engine;
 };
 
 namespace {
@@ -95,16 +162,22 @@ void engine_fail(gb_server* s, const std::string& msg) {
     s->cv_done.notify_all();
 }
 
-void engine_main(gb_server* s) {
+// One lane's engine: retire, admit, launch, wait, on the lane's own slots and stream.
+void engine_main(gb_server* s, uint32_t lane) {
     if (hipSetDevice(s->device) != hipSuccess) {
         engine_fail(s, "serving engine: hipSetDevice failed");
         return;
     }
-    std::vector<Query> active;
-    active.reserve(s->slots);
-    uint64_t seen = 0, left = 0;   // submissions counted and queries left queued at the last admission
+    Lane& L = s->lanes[lane];
+    std::vector<SlotInfo> info(L.size, SlotInfo{0, 0, 0, 0});
+    FreeSlots free_slots(L.size);
+    std::vector<Query> admit;
+    std::vector<uint32_t> admitted;
+    std::vector<gb_result> results;
+    uint32_t hi = 0, resident = 0;   // the lane's slots [0, hi) hold every query resident in it
+    uint64_t seen = 0;               // submissions counted at the last admission
     for (;;) {
-        if (active.empty() && left == 0) {
+        if (resident == 0 && lane == 0) {
             // idle: poll for arrivals for a short while, so a query that arrives now does not
             // wait for the thread to be woken; then sleep on the condition variable
             const uint64_t t0 = now_ns();
@@ -112,65 +185,83 @@ void engine_main(gb_server* s) {
                    now_ns() - t0 < IDLE_SPIN_NS) {
             }
         }
+        admit.clear();
         {
+            // lane 0 serves arrivals; lane 1 sleeps until lane 0 leaves queries queued (a burst)
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !active.empty(); });
+            (lane == 0 ? s->cv_work : s->cv_help).wait(lk, [&] { return s->stop || !s->queue.empty() || resident > 0; });
             seen = s->arrivals.load(std::memory_order_relaxed);
             if (s->stop) return;
-            while (active.size() < s->slots && !s->queue.empty()) {
-                active.push_back(s->queue.front());
-                s->queue.pop_front();
-            }
+            const size_t take = std::min(s->queue.size(), free_slots.size());
+            admit.assign(s->queue.v.begin() + (ptrdiff_t)s->queue.head,
+                         s->queue.v.begin() + (ptrdiff_t)(s->queue.head + take));
+            s->queue.head += take;
+            s->queue.compact();
             s->st.queued = s->queue.size();
-            s->st.resident = active.size();
-            left = s->queue.size();
+            s->lane_resident[lane] = resident + take;
+            s->st.resident = s->lane_resident[0] + s->lane_resident[1];
+            s->st.max_resident = std::max<uint64_t>(s->st.max_resident, s->st.resident);
+            if (lane == 0 && !s->queue.empty()) s->cv_help.notify_one();   // the other lane has room
         }
-        const uint32_t m = (uint32_t)active.size();
-        for (uint32_t i = 0; i < m; ++i) s->h_in[i] = ServeIn{active[i].n, active[i].k};
-        // the step reads its inputs from and writes its results to the pinned staging directly:
-        // a few bytes per query, so no copy commands sit between the launch and the answer
-        hipError_t e = launch_serve_step(s->stream, s->T, s->h_in, m, s->bound, s->chunks, s->h_out);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        admitted.clear();
+        for (const Query& q : admit) {
+            const uint32_t i = free_slots.take();
+            info[i] = SlotInfo{q.id, q.n, q.submit_ns, 0};
+            L.adm[i] = q.n;
+            admitted.push_back(i);
+            hi = std::max(hi, i + 1);
+        }
+        resident += (uint32_t)admit.size();
+        if (resident == 0) continue;
+        const uint32_t m = hi;
+        hipError_t e = launch_serve_step(L.stream, s->T, L.adm, L.d_slots, m, s->bound, s->chunks, L.done);
+        if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
         if (e != hipSuccess) {
             engine_fail(s, std::string("serving engine: ") + hipGetErrorString(e));
             return;
         }
         const uint64_t t = now_ns();
-        uint32_t keep = 0;
-        std::lock_guard<std::mutex> lk(s->mu);
+        for (uint32_t i : admitted) L.adm[i] = 0;
+        results.clear();
         for (uint32_t i = 0; i < m; ++i) {
-            Query q = active[i];
+            SlotInfo& q = info[i];
+            if (!q.n) continue;
             q.steps += 1;
-            if (s->h_out[i].state) {
-                gb_result r;
-                r.id = q.id;
-                r.n = q.n;
-                r.p = s->h_out[i].p;
-                r.submit_ns = q.submit_ns;
-                r.done_ns = t;
-                r.steps = q.steps;
-                r.reserved = 0;
-                s->done.push_back(r);
-            } else {
-                q.k = s->h_out[i].k;
-                active[keep++] = q;
-            }
+            const ServeDone d = L.done[i];
+            if (d.state != 1) continue;
+            results.push_back(gb_result{q.id, q.n, d.p, q.submit_ns, t, q.steps, 0});
+            q.n = 0;
+            free_slots.give(i);
         }
-        active.resize(keep);
+        const uint64_t finished = results.size();
+        const uint64_t in_step = resident;
+        resident -= (uint32_t)finished;
+        while (hi > 0 && info[hi - 1].n == 0) --hi;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->done.empty()) {
+            s->done.v.swap(results);   // hand the results over without copying them
+            s->done.head = 0;
+        } else {
+            s->done.v.insert(s->done.v.end(), results.begin(), results.end());
+        }
         s->st.steps += 1;
-        s->st.step_queries += m;
-        s->st.max_resident = std::max<uint64_t>(s->st.max_resident, m);
-        s->st.completed += m - keep;
-        s->st.resident = keep;
-        if (keep < m) s->cv_done.notify_all();
+        s->st.step_queries += in_step;
+        s->st.completed += finished;
+        s->lane_resident[lane] = resident;
+        s->st.resident = s->lane_resident[0] + s->lane_resident[1];
+        if (finished) s->cv_done.notify_all();
     }
 }
 
 void free_server(gb_server* s) {
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
-    if (s->h_in) (void)hipHostFree(s->h_in);
-    if (s->h_out) (void)hipHostFree(s->h_out);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    for (Lane& L : s->lanes)
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+    for (Lane& L : s->lanes) {
+        if (L.adm) (void)hipHostFree(L.adm);
+        if (L.done) (void)hipHostFree(L.done);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    if (s->d_slots) dfree(s->ctx, s->d_slots);
     delete s;
 }
 
@@ -186,8 +277,8 @@ int gb_serve_open(gb_ctx* c, const gb_serve_config* cfg, gb_server** out) {
     gb_serve_config k{};
     if (cfg) k = *cfg;
     uint64_t slots = k.slots ? k.slots : c->cfg.serve_slots;
-    if (k.mem_budget) slots = std::min<uint64_t>(slots, k.mem_budget / SLOT_BYTES);
-    if (slots == 0) return fail(c, GB_ENOMEM, "serving: the memory budget holds no slot");
+    if (k.mem_budget) slots = std::min<uint64_t>(slots, k.mem_budget / SLOT_DEVICE_BYTES);
+    if (slots == 0) return fail(c, GB_ENOMEM, "serving: the device memory budget holds no slot");
     if (slots > (1u << 24)) return fail(c, GB_EINVAL, "serving: more than 2^24 slots");
     HCK(c, hipSetDevice(c->device));
     ItemTable it;
@@ -203,21 +294,42 @@ int gb_serve_open(gb_ctx* c, const gb_serve_config* cfg, gb_server** out) {
     s->chunks = k.chunks_per_step ? k.chunks_per_step : DEFAULT_CHUNKS;
     s->queue_cap = k.queue_cap ? k.queue_cap : DEFAULT_QUEUE;
     s->st.slots = s->slots;
-    hipError_t e = hipStreamCreate(&s->stream);
-    if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void**>(&s->h_in), slots * sizeof(ServeIn), hipHostMallocDefault);
-    if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out), slots * sizeof(ServeOut), hipHostMallocDefault);
+    hipError_t e = dalloc(c, &s->d_slots, slots);   // the context's budget counts the slot state
+    const uint32_t half = (uint32_t)((slots + 1) / 2);
+    for (uint32_t l = 0; l < 2; ++l) {
+        Lane& L = s->lanes[l];
+        L.size = l == 0 ? half : (uint32_t)slots - half;
+        const size_t n = std::max<uint32_t>(L.size, 1);
+        if (e == hipSuccess) e = hipStreamCreate(&L.stream);
+        if (e == hipSuccess) L.d_slots = s->d_slots + (l == 0 ? 0 : half);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.adm), n * sizeof(uint64_t), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&L.done), n * sizeof(ServeDone), hipHostMallocDefault);
+        if (e == hipSuccess) std::memset(L.adm, 0, n * sizeof(uint64_t));
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_slots, 0, slots * sizeof(ServeSlot), s->lanes[0].stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->lanes[0].stream);
     if (e != hipSuccess) {
+        const int rc = hip_fail(c, e, __FILE__, __LINE__);
         free_server(s);
-        return fail(c, e == hipErrorOutOfMemory ? GB_ENOMEM : GB_EDEVICE,
-                    std::string("serving: set-up failed -- ") + hipGetErrorString(e));
+        c->err = "serving: set-up failed -- " + c->err;
+        return rc;
     }
     try {
-        s->engine = std::thread(engine_main, s);
+        for (uint32_t l = 0; l < 2; ++l)
+            if (s->lanes[l].size) s->lanes[l].engine = std::thread(engine_main, s, l);
     } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+            s->stopping.store(true);
+        }
+        s->cv_work.notify_all();
+        s->cv_help.notify_all();
+        for (Lane& L : s->lanes)
+            if (L.engine.joinable()) L.engine.join();
         free_server(s);
-        return fail(c, GB_EDEVICE, "serving: could not start the engine thread");
+        return fail(c, GB_EDEVICE, "serving: could not start the engine threads");
     }
     c->servers.fetch_add(1);
     *out = s;
@@ -234,18 +346,33 @@ int gb_serve_submit(gb_server* s, const uint64_t* n, uint32_t count, uint64_t* f
             return GB_EINVAL;
         }
     const uint64_t t = now_ns();
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->failed) return GB_EDEVICE;
-    if (s->queue.size() + count > s->queue_cap) {
-        s->err = "gb_serve_submit: queue full";
-        return GB_EAGAIN;
+    std::lock_guard<std::mutex> sk(s->submit_mu);   // ids of one submission stay consecutive
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->failed) return GB_EDEVICE;
+        // only the engine takes from the queue meanwhile: room now is room for every chunk below
+        if (s->queue.size() + count > s->queue_cap) {
+            s->err = "gb_serve_submit: queue full";
+            return GB_EAGAIN;
+        }
+        if (first_id) *first_id = s->next_id;
+        s->queue.v.reserve(s->queue.v.size() + count);   // one reallocation, not one per growth step
     }
-    if (first_id) *first_id = s->next_id;
-    for (uint32_t i = 0; i < count; ++i) s->queue.push_back(Query{s->next_id++, n[i], 0, t, 0});
-    s->arrivals.fetch_add(count, std::memory_order_release);
-    s->st.submitted += count;
-    s->st.queued = s->queue.size();
-    if (count) s->cv_work.notify_one();
+    // appended in chunks, so the engine starts on the first while the rest is copied
+    constexpr uint32_t CHUNK = 1u << 15;
+    for (uint32_t c0 = 0; c0 < count; c0 += CHUNK) {
+        const uint32_t c = std::min(CHUNK, count - c0);
+        std::lock_guard<std::mutex> lk(s->mu);
+        const size_t at = s->queue.v.size();
+        s->queue.v.resize(at + c);
+        Query* q = s->queue.v.data() + at;
+        for (uint32_t i = 0; i < c; ++i) q[i] = Query{s->next_id + i, n[c0 + i], t};
+        s->next_id += c;
+        s->arrivals.fetch_add(c, std::memory_order_release);
+        s->st.submitted += c;
+        s->st.queued = s->queue.size();
+        s->cv_work.notify_one();
+    }
     return GB_OK;
 }
 
@@ -253,16 +380,26 @@ int gb_serve_poll(gb_server* s, gb_result* out, uint32_t max, uint32_t* got, uin
     if (!s) return GB_EINVAL;
     if (got) *got = 0;
     if (max && !out) return GB_EINVAL;
-    std::unique_lock<std::mutex> lk(s->mu);
-    if (wait_ms && s->done.empty() && !s->failed)
-        s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !s->done.empty() || s->failed; });
-    uint32_t k = 0;
-    while (k < max && !s->done.empty()) {
-        out[k++] = s->done.front();
-        s->done.pop_front();
+    // pollers take the engines' finished results whole (a swap under the engines' lock) and copy
+    // them out under their own lock, so an engine never waits for a poller's copy
+    std::lock_guard<std::mutex> pk(s->poll_mu);
+    Fifo<gb_result>& mine = s->polled;
+    bool failed = false;
+    if (mine.empty()) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (wait_ms && s->done.empty() && !s->failed)
+            s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !s->done.empty() || s->failed; });
+        mine.v.clear();
+        mine.head = 0;
+        mine.v.swap(s->done.v);
+        std::swap(mine.head, s->done.head);
+        failed = s->failed;
     }
+    const uint32_t k = (uint32_t)std::min<size_t>(max, mine.size());
+    if (k) std::memcpy(out, mine.v.data() + mine.head, k * sizeof(gb_result));
+    mine.head += k;
     if (got) *got = k;
-    if (s->failed && k == 0) return GB_EDEVICE;
+    if (failed && k == 0) return GB_EDEVICE;
     return GB_OK;
 }
 
@@ -281,7 +418,9 @@ int gb_serve_close(gb_server* s) {
         s->stopping.store(true, std::memory_order_release);
     }
     s->cv_work.notify_all();
-    if (s->engine.joinable()) s->engine.join();
+    s->cv_help.notify_all();
+    for (Lane& L : s->lanes)
+        if (L.engine.joinable()) L.engine.join();
     gb_ctx* c = s->ctx;
     (void)hipSetDevice(s->device);
     free_server(s);

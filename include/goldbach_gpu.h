@@ -64,8 +64,10 @@ typedef struct gb_config {
                                 device allocation of the context counts against it and a call
                                 that would pass it returns GB_ENOMEM */
     uint32_t serve_slots;    /* serving slots (default 65536) */
-    uint32_t reserved;
+    uint32_t flags;          /* GB_CFG_* (0: none) */
 } gb_config;
+
+#define GB_CFG_KERNEL_TIMES 0x1u   /* time the plain form's sieve and marking launches apart (gb_kernel_times_get) */
 
 /* Per segment / per range / per window results. */
 typedef struct gb_seg_stats {
@@ -128,6 +130,23 @@ int gb_verify_segment(gb_ctx *ctx, uint64_t seg_start, uint64_t seg_even_count, 
 int gb_verify_segment_noalloc(gb_ctx *ctx, uint64_t seg_start, uint64_t seg_even_count, uint64_t p_small,
                               uint32_t flags, uint64_t *verified_bits_out, uint32_t *pmin_out,
                               gb_seg_stats *stats);
+
+/* Device time of the plain form's launches, per kernel, summed over the
+ * segments run since the last reset (contexts opened with
+ * GB_CFG_KERNEL_TIMES; zeros otherwise).  No counterpart in the reference,
+ * which publishes no per-kernel split (benchmarks/bench_5.sh:30-34 parses
+ * lines no binary prints).  reset != 0 zeroes the sums after reading. */
+typedef struct gb_kernel_times {
+    uint64_t segments;      /* plain segments timed */
+    uint64_t window_bits;   /* window bits the sieve launches wrote */
+    uint64_t sieve_primes;  /* sieving-prime entries (16 B each) they read */
+    uint64_t evens;         /* even n the marking launches covered */
+    double   sieve_ms;      /* the window sieve (goldbach.cu:198-259's job) */
+    double   mark_ms;       /* Phase-1 marking (goldbach.cu:288-311) */
+    double   other_ms;      /* the segment's other launches: prime entries, slow path, reduction */
+} gb_kernel_times;
+
+int gb_kernel_times_get(gb_ctx *ctx, gb_kernel_times *out, int reset);
 
 /* ---- whole ranges ------------------------------------------------------ */
 
@@ -306,8 +325,9 @@ int gb_count_partitions(gb_ctx *ctx, uint64_t start, uint64_t count, uint64_t *c
 /* ---- serving loop with continuous batching ---------------------------- */
 
 /* A serving engine on one context: a host thread that keeps up to `slots`
- * queries resident on the device and runs one step kernel after another
- * over all of them.  Each step advances every resident query by up to
+ * queries resident on the device (their state in device memory, so the
+ * device memory granted to the server bounds admission) and runs one step
+ * kernel after another over all of them.  Each step advances every resident query by up to
  * `chunks_per_step` chunks of 64 candidate primes; queries that finish leave
  * their slot at the step boundary and queued queries take the free slots for
  * the next step, so arrivals join work already on the device instead of
@@ -319,8 +339,9 @@ typedef struct gb_server gb_server;
 typedef struct gb_serve_config {
     uint32_t slots;            /* resident queries per step (0 = the context's serve_slots) */
     uint32_t chunks_per_step;  /* 64-prime chunks per query per step (0 = 4) */
-    uint64_t mem_budget;       /* bytes of slot state (40 per slot, pinned host memory the step
-                                  kernel reads and writes); slots shrink to fit (0 = no limit) */
+    uint64_t mem_budget;       /* device bytes of query state the server may hold (16 per slot,
+                                  allocated from the context's budget); slots shrink to fit
+                                  (0 = no limit beyond the context's budget) */
     uint64_t queue_cap;        /* queries waiting for a slot before submit says GB_EAGAIN (0 = 2^22) */
 } gb_serve_config;
 

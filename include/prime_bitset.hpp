@@ -9,8 +9,9 @@
 //   builder only clears composites <= limit)
 //
 // build_prime_bitset here gives every worker thread whole 64-bit words, so no
-// word is ever written by two threads.  A GPU-built variant with the same
-// layout is goldbach::gpu::build_prime_bitset in prime_bitset_gpu.hpp.
+// word is ever written by two threads.  The same words built on the GPU come
+// from the C ABI: gb_sieve_window(ctx, 3, q_high, words) with q_high the
+// largest odd number <= limit (include/goldbach_gpu.h).
 #pragma once
 
 #include <cassert>

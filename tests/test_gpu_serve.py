@@ -101,16 +101,24 @@ def test_slots_bound_residency(ctx, orc):
     assert st["slots"] == 8 and st["max_resident"] <= 8 and st["steps"] >= 1000 // 8
 
 
-def test_memory_budget_admission(ctx):
+def test_memory_budget_admission(ctx, orc):
     gb = load_pkg()
-    # 40 device bytes of slot state per query: a 200-byte budget admits 5 at a time
-    with ctx.server(slots=1000, mem_budget=200) as srv:
+    # 16 bytes of device memory per resident query: an 80-byte budget admits 5 at a time
+    ns = serving_ns(50, seed=9)
+    with ctx.server(slots=1000, mem_budget=80) as srv:
         assert srv.stats()["slots"] == 5
-        srv.run(serving_ns(50, seed=9))
+        assert np.array_equal(srv.run(ns), orc.min_partition_batch(ns))
         assert srv.stats()["max_resident"] <= 5
     with pytest.raises(gb.GoldbachError) as e:
-        ctx.server(mem_budget=39)
+        ctx.server(mem_budget=15)
     assert e.value.status == gb.GB_ENOMEM
+    # the slot state comes out of the context's own device budget
+    with gb.Context(0, mem_budget=256 << 20) as c:
+        with pytest.raises(gb.GoldbachError) as e:
+            c.server(slots=1 << 24)                      # 256 MiB of slot state
+        assert e.value.status == gb.GB_ENOMEM and "budget" in e.value.message
+        with c.server(slots=1 << 16) as srv:
+            assert list(srv.run([98, 100])) == [19, 3]
 
 
 def test_submit_contract(ctx):
