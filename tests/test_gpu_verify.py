@@ -211,8 +211,10 @@ def _range_fixtures():
 
 @pytest.mark.parametrize("form", ["batched", "plain"])
 def test_range_sums_equal_fixture(ctx, gb, form):
-    # [4, 1e9], [4, 1e10] and [1e10, 1e10 + 2^31] against the oracle's statistics; their
-    # largest p_min are published records (721013438: 1789, 7473202036: 1877, 11001080372: 1879)
+    # [4, 1e9], [4, 1e10], [1e10, 1e10 + 2^31], [1e12 - 2^31, 1e12] and the whole headline range
+    # [4, 1e12] against the oracle's statistics (sum, maximum and argmax of p_min, Phase-2 and
+    # failure counts); the maxima from 4 are published records (721013438: 1789,
+    # 7473202036: 1877, 599533546358: 3457)
     f = gb.GB_FORM_BATCHED if form == "batched" else gb.GB_FORM_PLAIN
     for r in _range_fixtures():
         st = ctx.verify_range(r["start"], r["limit"], r["p_small"], form=f, want_pmin=True)
