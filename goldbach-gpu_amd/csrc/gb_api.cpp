@@ -938,6 +938,7 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
         WinDesc& wd = c->h_wins[w];
         wd = WinDesc{pcs[w].start, pcs[w].evens, 0, 0, 0, 0, 0, 0, 0};
         const uint64_t nt = piece_tiles(sh, wd.start, wd.evens);
+        if (sh.wheel && nt > CHUNK_TILES) return fail(c, GB_EINVAL, "wheel piece longer than a launch's tiles");
         wd.tile0 = tiles;
         c->h_lohi[w] = tiles;
         tiles += nt;
@@ -990,7 +991,7 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
         // the residue launches cover the whole table: a piece's own count (read on the device)
         // bounds the work, so chunks anywhere in the range share the graph
         if (sh.wheel) {
-            if (e == hipSuccess) e = launch_residues_wheel(s, T, c->d_wins, np, c->nsp, c->d_r0);
+            if (e == hipSuccess) e = launch_residues_wheel(s, T, c->d_wins, np, c->nsp, cls_len, c->d_r0);
             if (e == hipSuccess)
                 e = launch_wheel(s, T, mp, c->d_wins, np, tiles, cls_len, (uint32_t)grid, sh.wg, c->d_counter, sl,
                                  c->d_rec, c->d_r0);

@@ -814,9 +814,20 @@ __device__ __forceinline__ uint32_t wheel_p2(uint32_t off, uint32_t p, uint32_t 
     return i >= end ? end : (uint32_t)i;
 }
 
+// (e - t lp) mod p for a tile t stream lengths above the window's base (WheelRes): 32-bit and
+// fp32 arithmetic only, exact for t < 2^17.
+__device__ __forceinline__ uint32_t wheel_off(const WheelRes& E, uint32_t t, float tf) {
+    const uint32_t q = (uint32_t)(tf * E.ratio);                 // t lp / p, less one at most, or plus one
+    uint32_t r = __umul24(t, E.lp) - __umul24(q, E.p);           // t lp mod p + {-p, 0, p}, wrapped
+    r = min(r, r + E.p);                                         // negative: add p
+    r = min(r, r - E.p);                                         // >= p: subtract p
+    const uint32_t o = E.e - r;
+    return min(o, o + E.p);                                      // e < r: add p
+}
+
 template <int BS, uint32_t SW>
 __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict__ med0, int64_t K0, uint32_t nbits,
-                                 const Tables& T, uint32_t tid, const PrimeRes* __restrict__ pr, int64_t kbase) {
+                                 const Tables& T, uint32_t tid, const WheelRes* __restrict__ pr, uint32_t tl) {
     const uint32_t nw64 = (nbits + 63) >> 6;
     // 1) primes 5..61 as per-word masks in each stream; bits standing for q <= 1 are zero
 #pragma unroll
@@ -853,10 +864,10 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
     const uint64_t q_top = 6ull * (uint64_t)(K0 + (int64_t)nbits - 1) + 5;
     const uint32_t np = count_le(T.sp, T.nsp, isqrt64(q_top));
     const uint32_t kmed_end = np < T.n_med ? np : T.n_med;
-    const double dbf = (double)(K0 - kbase);
+    const float tf = (float)tl;
     for (uint32_t k = NPAT + tid; k < kmed_end; k += BS) {
-        const PrimeRes e = pr[k];
-        const uint32_t o1 = first_off((double)e.e - dbf, e.p, e.inv);
+        const WheelRes e = pr[k];
+        const uint32_t o1 = wheel_off(e, tl, tf);
         uint32_t o5 = o1 + T.d5[k];
         o5 = min(o5, o5 - e.p);
         med0[2 * (k - NPAT)] = wheel_p2(o1, e.p, 1, K0, nbits);
@@ -923,20 +934,20 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
     {
         const int64_t kroot = (int64_t)(isqrt64(6ull * (uint64_t)(K0 > 0 ? K0 : 0) + 1) );
         uint32_t k = T.n_med + tid;
-        PrimeRes nx{0, 0, 0.0};
+        WheelRes nx{0, 0, 0, 0.0f};
         uint32_t d_n = 0;
         if (k < np) {
             nx = ld32(pr, k);
             d_n = ld32(T.d5, k);
         }
         while (k < np) {
-            const PrimeRes e = nx;
+            const WheelRes e = nx;
             const uint32_t d5 = d_n, kn = k + BS;
             if (kn < np) {
                 nx = ld32(pr, kn);
                 d_n = ld32(T.d5, kn);
             }
-            uint32_t o1 = first_off((double)e.e - dbf, e.p, e.inv);
+            uint32_t o1 = wheel_off(e, tl, tf);
             uint32_t o5 = o1 + d5;
             o5 = min(o5, o5 - e.p);
             if ((int64_t)e.p > kroot) {   // p^2 may lie above the tile's first q
@@ -1138,7 +1149,7 @@ template <int BS, uint32_t CMAX>
 __global__ __launch_bounds__(BS, 4) void wheel_kernel(Tables T, MarkParams mp, const WinDesc* __restrict__ wins,
                                                    uint32_t nwin, uint64_t ntiles, uint32_t cls_len,
                                                    uint64_t* __restrict__ tile_counter, SlowList sl,
-                                                   StatRec* __restrict__ rec, const PrimeRes* __restrict__ pr) {
+                                                   StatRec* __restrict__ rec, const WheelRes* __restrict__ pr) {
     constexpr uint32_t SW = wh_stream_words(CMAX);
     __shared__ __align__(16) uint32_t wh[2 * SW];   // [S1 | S5] at fixed LDS addresses
     extern __shared__ uint32_t lds[];                               // [stats | tile id | medium first marks]
@@ -1165,7 +1176,7 @@ __global__ __launch_bounds__(BS, 4) void wheel_kernel(Tables T, MarkParams mp, c
         const uint64_t n_lo = wd.start, n_hi = wd.start + 2 * (wd.evens - 1);
         const uint64_t n0 = (n_lo - n_lo % 6) + 6ull * cls_len * (t - wd.tile0);
         const int64_t K0 = (int64_t)(n0 / 6) - (int64_t)WH_HALO;
-        sieve_wheel_tile<BS, SW>(wh, med0, K0, nbits, T, tid, pr + wd.r0off, wd.kbase);
+        sieve_wheel_tile<BS, SW>(wh, med0, K0, nbits, T, tid, pr + wd.r0off, (uint32_t)(t - wd.tile0));
         LaneStats st = lane_stats_zero();
         uint32_t jlo0, jhi0, jlo2, jhi2, jlo4, jhi4;
         class_range(n0, 0, cls_len, n_lo, n_hi, jlo0, jhi0);
@@ -1184,15 +1195,18 @@ __global__ __launch_bounds__(BS, 4) void wheel_kernel(Tables T, MarkParams mp, c
     }
 }
 
-// pr[wd.r0off + k] for the wheel: e = first bit of S1 (q = 6(kbase + e) + 1) holding a multiple of p
-__global__ void residue_wheel_kernel(Tables T, const WinDesc* __restrict__ wins, PrimeRes* __restrict__ pr) {
+// pr[wd.r0off + k] for the wheel: e = first bit of S1 (q = 6(kbase + e) + 1) holding a multiple of
+// p, and the constants of the tile offset (cls_len mod p and its ratio to p)
+__global__ void residue_wheel_kernel(Tables T, const WinDesc* __restrict__ wins, uint32_t cls_len,
+                                     WheelRes* __restrict__ pr) {
     const WinDesc wd = wins[blockIdx.y];
     const uint64_t kb = (uint64_t)(wd.kbase + (int64_t)WH_HALO);   // >= 0
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < wd.np; k += gridDim.x * blockDim.x) {
         const uint32_t p = T.sp[k];
-        PrimeRes e;
+        WheelRes e;
         e.p = p;
-        e.inv = T.inv[k];
+        e.lp = cls_len % p;
+        e.ratio = (float)((double)e.lp / (double)p);
         if (p % 3 == 0) {
             e.e = 0;
         } else {
@@ -1631,11 +1645,12 @@ size_t wheel_lds_bytes(uint32_t n_med, uint32_t wg_per_cu) {
 }
 
 hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin,
-                                 uint32_t max_np, PrimeRes* pr) {
+                                 uint32_t max_np, uint32_t cls_len, PrimeRes* pr) {
     if (!nwin || !max_np) return hipSuccess;
     uint32_t bx = (max_np + 255) / 256;
     if (bx > 1024) bx = 1024;
-    hipLaunchKernelGGL(residue_wheel_kernel, dim3(bx, nwin), dim3(256), 0, s, T, wins, pr);
+    hipLaunchKernelGGL(residue_wheel_kernel, dim3(bx, nwin), dim3(256), 0, s, T, wins, cls_len,
+                       reinterpret_cast<WheelRes*>(pr));
     return hipGetLastError();
 }
 
@@ -1651,14 +1666,14 @@ hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, co
         hipError_t e = allow_lds(wheel_kernel<BS, WH_CLASS_HALF>, lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((wheel_kernel<BS, WH_CLASS_HALF>), dim3(grid), dim3(BS), lds, s, T, mp, wins, nwin, ntiles,
-                           cls_len, tile_counter, sl, rec, pr);
+                           cls_len, tile_counter, sl, rec, reinterpret_cast<const WheelRes*>(pr));
         return hipGetLastError();
     }
     const size_t lds = sizeof(LaneStats) * (FUSED_BS / 64) + 8 + (size_t)2 * med * 4;
     hipError_t e = allow_lds(wheel_kernel<FUSED_BS, WH_CLASS_MAX>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((wheel_kernel<FUSED_BS, WH_CLASS_MAX>), dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin,
-                       ntiles, cls_len, tile_counter, sl, rec, pr);
+                       ntiles, cls_len, tile_counter, sl, rec, reinterpret_cast<const WheelRes*>(pr));
     return hipGetLastError();
 }
 

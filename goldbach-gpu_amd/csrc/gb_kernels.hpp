@@ -60,6 +60,20 @@ struct PrimeRes {
     double inv;     // 1/p
 };
 
+// A sieving prime as the wheel form's window sees it, 16 bytes like PrimeRes.  The tiles of a
+// window lie a whole number t of stream lengths L (= cls_len) above its base, so a tile's first
+// S1 mark is (e - t L) mod p = (e - t lp) mod p with lp = L mod p, and t lp mod p comes from
+// 32-bit arithmetic: q = floor(t * ratio) in fp32 is off by at most one for t < 2^17
+// (ratio = lp / p rounded to fp32: relative error 2^-24), so t lp - q p lies in (-p, 2p) and
+// wraps exactly in 32 bits.
+struct WheelRes {
+    uint32_t p;
+    uint32_t e;      // first bit b >= 0 of S1 (q = 6(kbase + b) + 1) with p | q
+    uint32_t lp;     // L mod p
+    float ratio;     // lp / p
+};
+static_assert(sizeof(WheelRes) == sizeof(PrimeRes), "wheel entries share the PrimeRes buffer");
+
 // Sieving primes needed for windows whose largest q is q_top.
 __host__ __device__ inline uint64_t sieve_prime_bound(uint64_t q_top) {
     uint64_t r = 0;
@@ -107,10 +121,10 @@ constexpr uint32_t WH_CLASS_HALF = 1u << 18;         // ... with two half-size w
 // u32 per stream for tiles of at most cmax n per class
 __host__ __device__ constexpr uint32_t wh_stream_words(uint32_t cmax) { return (cmax + WH_HALO + 64) / 32 + 2; }
 size_t wheel_lds_bytes(uint32_t n_med, uint32_t wg_per_cu);
-// pr[wins[w].r0off + k]: prime k as window w's stream S1 sees it (e = first stream bit of a
-// multiple of p, counted from k = kbase).
+// pr[wins[w].r0off + k], as WheelRes: prime k as window w's stream S1 sees it (e = first stream
+// bit of a multiple of p, counted from k = kbase) for tiles of cls_len bits per stream.
 hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin,
-                                 uint32_t max_np, PrimeRes* pr);
+                                 uint32_t max_np, uint32_t cls_len, PrimeRes* pr);
 hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint32_t wg_per_cu, uint64_t* tile_counter,
                         SlowList sl, StatRec* rec, const PrimeRes* pr);
