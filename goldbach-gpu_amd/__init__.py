@@ -89,7 +89,8 @@ class GbServeConfig(ctypes.Structure):
 
 class GbServeStats(ctypes.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("steps", u64), ("step_queries", u64),
-                ("max_resident", u64), ("queued", u64), ("resident", u64), ("slots", u32), ("reserved", u32)]
+                ("max_resident", u64), ("queued", u64), ("resident", u64), ("slots", u32), ("reserved", u32),
+                ("queue_ns", u64), ("admit_ns", u64), ("device_ns", u64), ("retire_ns", u64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}

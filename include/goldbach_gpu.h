@@ -364,6 +364,11 @@ typedef struct gb_serve_stats {
     uint64_t resident;      /* on the device now */
     uint32_t slots;         /* slots after the memory budget */
     uint32_t reserved;
+    /* host time of the engine threads, summed over them (ns): */
+    uint64_t queue_ns;      /* taking queries from the queue and handing results over, lock waits included */
+    uint64_t admit_ns;      /* placing admitted queries in slots */
+    uint64_t device_ns;     /* from a step's launch until the host sees it finished */
+    uint64_t retire_ns;     /* reading the step's answers and building results */
 } gb_serve_stats;
 
 /* Starts the engine.  The context must stay open (gb_close refuses with
