@@ -299,7 +299,10 @@ int ensure_wins(gb_ctx* c, uint32_t n) {
 int ensure_rec(gb_ctx* c, uint64_t n) {
     if (n <= c->rec_cap) return GB_OK;
     dfree(c, c->d_rec);
+    dfree(c, c->d_tinfo);
+    c->rec_cap = 0;
     HCK(c, dalloc(c, &c->d_rec, n));
+    HCK(c, dalloc(c, &c->d_tinfo, n));
     c->rec_cap = n;
     return GB_OK;
 }
@@ -715,7 +718,7 @@ int gb_close(gb_ctx* c) {
     }
     void* bufs[] = {c->d_sp, c->d_inv, c->d_c32, c->d_d5, c->d_p2bits, c->d_window, c->d_verified, c->d_pmin, c->d_rec,
                     c->d_out, c->d_slow_n, c->d_slow_win, c->d_slow_ctl, c->d_slow_res, c->d_wins, c->d_rec_lo,
-                    c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0};
+                    c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0, c->d_tinfo};
     for (void* b : bufs)
         if (b) dfree(c, b);
     if (c->h_out) dhostfree(c->h_out);
@@ -960,12 +963,12 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
         }
         nres += wd.np;
     }
-    const void* old_bufs[] = {c->d_rec, c->d_r0};
+    const void* old_bufs[] = {c->d_rec, c->d_r0, c->d_tinfo};
     rc = ensure_rec(c, tiles);
     if (rc) return rc;
     rc = ensure_r0(c, std::max<uint64_t>(nres, c->nsp));
     if (rc) return rc;
-    if (old_bufs[0] != c->d_rec || old_bufs[1] != c->d_r0) drop_graphs(c);
+    if (old_bufs[0] != c->d_rec || old_bufs[1] != c->d_r0 || old_bufs[2] != c->d_tinfo) drop_graphs(c);
     HCK(c, hipMemcpyAsync(c->d_wins, c->h_wins, np * sizeof(WinDesc), hipMemcpyHostToDevice, s));
     HCK(c, hipMemcpyAsync(c->d_rec_lo, c->h_lohi, np * 8ull, hipMemcpyHostToDevice, s));
     HCK(c, hipMemcpyAsync(c->d_rec_hi, c->h_lohi + np, np * 8ull, hipMemcpyHostToDevice, s));
@@ -973,7 +976,8 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
     if (grid > tiles) grid = tiles;
     SlowList sl = slow_list(c);
     const std::vector<uint64_t> key = {sh.wheel ? 1ull : 0ull, sh.tile_log2, sh.wg, np, tiles, grid, p_small, flags,
-                                       (uint64_t)(uintptr_t)c->d_rec, (uint64_t)(uintptr_t)c->d_wins,
+                                       (uint64_t)(uintptr_t)c->d_rec, (uint64_t)(uintptr_t)c->d_tinfo,
+                                       (uint64_t)(uintptr_t)c->d_wins,
                                        (uint64_t)(uintptr_t)c->d_out, (uint64_t)(uintptr_t)c->d_r0,
                                        (uint64_t)(uintptr_t)c->d_rec_lo, (uint64_t)(uintptr_t)c->d_rec_hi};
     hipGraphExec_t exec = nullptr;
@@ -994,7 +998,7 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
             if (e == hipSuccess) e = launch_residues_wheel(s, T, c->d_wins, np, c->nsp, cls_len, c->d_r0);
             if (e == hipSuccess)
                 e = launch_wheel(s, T, mp, c->d_wins, np, tiles, cls_len, (uint32_t)grid, sh.wg, c->d_counter, sl,
-                                 c->d_rec, c->d_r0);
+                                 c->d_rec, c->d_r0, c->d_tinfo);
         } else {
             if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, np, c->nsp, c->d_r0);
             if (e == hipSuccess)

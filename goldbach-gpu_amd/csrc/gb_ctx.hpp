@@ -60,6 +60,7 @@ struct gb_ctx {
     uint32_t* d_pmin = nullptr;
     uint64_t pmin_cap = 0;
     gbk::StatRec* d_rec = nullptr;
+    uint64_t* d_tinfo = nullptr;   // wheel launches: per tile, window index and sieving-prime count (rec_cap entries)
     uint64_t rec_cap = 0;
     gbk::StatRec* d_out = nullptr;
     uint32_t out_cap = 0;

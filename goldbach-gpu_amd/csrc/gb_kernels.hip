@@ -827,7 +827,8 @@ __device__ __forceinline__ uint32_t wheel_off(const WheelRes& E, uint32_t t, flo
 
 template <int BS, uint32_t SW>
 __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict__ med0, int64_t K0, uint32_t nbits,
-                                 const Tables& T, uint32_t tid, const WheelRes* __restrict__ pr, uint32_t tl) {
+                                 const Tables& T, uint32_t tid, const WheelRes* __restrict__ pr, uint32_t tl,
+                                 uint32_t np) {
     const uint32_t nw64 = (nbits + 63) >> 6;
     // 1) primes 5..61 as per-word masks in each stream; bits standing for q <= 1 are zero
 #pragma unroll
@@ -860,9 +861,8 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
             sw[k] = w;
         }
     }
-    // 2) first marks of the medium primes in both streams
-    const uint64_t q_top = 6ull * (uint64_t)(K0 + (int64_t)nbits - 1) + 5;
-    const uint32_t np = count_le(T.sp, T.nsp, isqrt64(q_top));
+    // 2) first marks of the medium primes in both streams (np: the primes up to the square root
+    //    of the tile's last q, from wheel_tiles_kernel)
     const uint32_t kmed_end = np < T.n_med ? np : T.n_med;
     const float tf = (float)tl;
     for (uint32_t k = NPAT + tid; k < kmed_end; k += BS) {
@@ -1145,11 +1145,37 @@ __device__ __forceinline__ void class_range(uint64_t n0, uint32_t c, uint32_t le
     if (jhi < jlo) jhi = jlo;
 }
 
+// Tile t of a wheel launch (n0 = the window's first multiple of 6 below its start + 6 cls_len
+// (t - tile0)) sieves stream bits [0, cls_len + WH_HALO + 64) from k = n0/6 - WH_HALO; its
+// last q is 6(K0 + nbits - 1) + 5.
+__device__ __forceinline__ uint64_t wheel_tile_n0(const WinDesc& wd, uint64_t t, uint32_t cls_len) {
+    return (wd.start - wd.start % 6) + 6ull * cls_len * (t - wd.tile0);
+}
+
+// info[t] = (window index << 32) | number of sieving primes up to the square root of the tile's
+// last q: the two binary searches of a tile, done once per launch for all tiles in parallel.
+__global__ void wheel_tiles_kernel(Tables T, const WinDesc* __restrict__ wins, uint32_t nwin, uint64_t ntiles,
+                                   uint32_t cls_len, uint64_t* __restrict__ info) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    uint32_t lo = 0, hi = nwin;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (wins[mid].tile0 <= t) lo = mid; else hi = mid;
+    }
+    const WinDesc wd = wins[lo];
+    const uint32_t nbits = cls_len + WH_HALO + 64;
+    const int64_t K0 = (int64_t)(wheel_tile_n0(wd, t, cls_len) / 6) - (int64_t)WH_HALO;
+    const uint64_t q_top = 6ull * (uint64_t)(K0 + (int64_t)nbits - 1) + 5;
+    info[t] = ((uint64_t)lo << 32) | count_le(T.sp, T.nsp, isqrt64(q_top));
+}
+
 template <int BS, uint32_t CMAX>
 __global__ __launch_bounds__(BS, 4) void wheel_kernel(Tables T, MarkParams mp, const WinDesc* __restrict__ wins,
-                                                   uint32_t nwin, uint64_t ntiles, uint32_t cls_len,
+                                                   uint64_t ntiles, uint32_t cls_len,
                                                    uint64_t* __restrict__ tile_counter, SlowList sl,
-                                                   StatRec* __restrict__ rec, const WheelRes* __restrict__ pr) {
+                                                   StatRec* __restrict__ rec, const WheelRes* __restrict__ pr,
+                                                   const uint64_t* __restrict__ tinfo) {
     constexpr uint32_t SW = wh_stream_words(CMAX);
     __shared__ __align__(16) uint32_t wh[2 * SW];   // [S1 | S5] at fixed LDS addresses
     extern __shared__ uint32_t lds[];                               // [stats | tile id | medium first marks]
@@ -1167,16 +1193,13 @@ __global__ __launch_bounds__(BS, 4) void wheel_kernel(Tables T, MarkParams mp, c
         const uint64_t t = s_tile;
         __syncthreads();
         if (t >= ntiles) break;
-        uint32_t lo = 0, hi = nwin;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (wins[mid].tile0 <= t) lo = mid; else hi = mid;
-        }
+        const uint64_t ti = tinfo[t];
+        const uint32_t lo = (uint32_t)(ti >> 32);
         const WinDesc wd = wins[lo];
         const uint64_t n_lo = wd.start, n_hi = wd.start + 2 * (wd.evens - 1);
-        const uint64_t n0 = (n_lo - n_lo % 6) + 6ull * cls_len * (t - wd.tile0);
+        const uint64_t n0 = wheel_tile_n0(wd, t, cls_len);
         const int64_t K0 = (int64_t)(n0 / 6) - (int64_t)WH_HALO;
-        sieve_wheel_tile<BS, SW>(wh, med0, K0, nbits, T, tid, pr + wd.r0off, (uint32_t)(t - wd.tile0));
+        sieve_wheel_tile<BS, SW>(wh, med0, K0, nbits, T, tid, pr + wd.r0off, (uint32_t)(t - wd.tile0), (uint32_t)ti);
         LaneStats st = lane_stats_zero();
         uint32_t jlo0, jhi0, jlo2, jhi2, jlo4, jhi4;
         class_range(n0, 0, cls_len, n_lo, n_hi, jlo0, jhi0);
@@ -1656,8 +1679,12 @@ hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* 
 
 hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint32_t wg_per_cu, uint64_t* tile_counter,
-                        SlowList sl, StatRec* rec, const PrimeRes* pr) {
+                        SlowList sl, StatRec* rec, const PrimeRes* pr, uint64_t* tinfo) {
     if (cls_len > WH_CLASS_MAX || cls_len < 64 || (cls_len & 63)) return hipErrorInvalidValue;
+    if (!ntiles) return hipSuccess;
+    hipLaunchKernelGGL(wheel_tiles_kernel, dim3((uint32_t)((ntiles + 255) / 256)), dim3(256), 0, s, T, wins, nwin,
+                       ntiles, cls_len, tinfo);
+    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
     const uint32_t med = T.n_med > NPAT ? T.n_med - NPAT : 1;
     if (wg_per_cu >= 2) {   // two half-size workgroups per CU: one's LDS phase can overlap the other's VALU phase
         if (cls_len > WH_CLASS_HALF) return hipErrorInvalidValue;
@@ -1665,15 +1692,15 @@ hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, co
         const size_t lds = sizeof(LaneStats) * (BS / 64) + 8 + (size_t)2 * med * 4;
         hipError_t e = allow_lds(wheel_kernel<BS, WH_CLASS_HALF>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((wheel_kernel<BS, WH_CLASS_HALF>), dim3(grid), dim3(BS), lds, s, T, mp, wins, nwin, ntiles,
-                           cls_len, tile_counter, sl, rec, reinterpret_cast<const WheelRes*>(pr));
+        hipLaunchKernelGGL((wheel_kernel<BS, WH_CLASS_HALF>), dim3(grid), dim3(BS), lds, s, T, mp, wins, ntiles,
+                           cls_len, tile_counter, sl, rec, reinterpret_cast<const WheelRes*>(pr), tinfo);
         return hipGetLastError();
     }
     const size_t lds = sizeof(LaneStats) * (FUSED_BS / 64) + 8 + (size_t)2 * med * 4;
     hipError_t e = allow_lds(wheel_kernel<FUSED_BS, WH_CLASS_MAX>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((wheel_kernel<FUSED_BS, WH_CLASS_MAX>), dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin,
-                       ntiles, cls_len, tile_counter, sl, rec, reinterpret_cast<const WheelRes*>(pr));
+    hipLaunchKernelGGL((wheel_kernel<FUSED_BS, WH_CLASS_MAX>), dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins,
+                       ntiles, cls_len, tile_counter, sl, rec, reinterpret_cast<const WheelRes*>(pr), tinfo);
     return hipGetLastError();
 }
 

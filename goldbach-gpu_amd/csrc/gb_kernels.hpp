@@ -125,9 +125,11 @@ size_t wheel_lds_bytes(uint32_t n_med, uint32_t wg_per_cu);
 // bit of a multiple of p, counted from k = kbase) for tiles of cls_len bits per stream.
 hipError_t launch_residues_wheel(hipStream_t s, const Tables& T, const WinDesc* wins, uint32_t nwin,
                                  uint32_t max_np, uint32_t cls_len, PrimeRes* pr);
+// The wheel launch: first each tile's window and sieving-prime count into tinfo[ntiles], then
+// the persistent wheel kernel over the tiles.
 hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t cls_len, uint32_t grid, uint32_t wg_per_cu, uint64_t* tile_counter,
-                        SlowList sl, StatRec* rec, const PrimeRes* pr);
+                        SlowList sl, StatRec* rec, const PrimeRes* pr, uint64_t* tinfo);
 
 // Resolve slow-list entries: search primes above sp[K-1] up to bound.
 hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
