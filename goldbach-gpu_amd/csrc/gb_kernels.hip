@@ -1514,6 +1514,53 @@ __device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint6
     return 0;
 }
 
+// partition_chunk with the strong-test bases spread over the workgroup's SERVE_BASES waves:
+// wave w tests every candidate of the chunk to base w (those q whose base count is below w+1
+// pass it), and a candidate is prime when it passes in every wave, so the chunk's answer takes
+// the time of one strong test instead of base 2 followed by the others.  pass[SERVE_BASES] is
+// the workgroup's LDS.  Returns as partition_chunk; uniform over the workgroup.
+__device__ int partition_chunk_wide(const Tables& T, uint64_t n, uint64_t half, uint64_t bound, uint64_t base,
+                                    uint32_t lane, uint32_t w, uint64_t* pass, uint64_t& res) {
+    const uint64_t k = base + lane;
+    uint64_t p;
+    bool isp;
+    if (k < T.nsp) {
+        p = T.sp[k];
+        isp = true;
+    } else {
+        p = (uint64_t)T.sp[T.nsp - 1] + 2 * (k - T.nsp + 1);
+        isp = p <= half && p <= bound && is_prime_u64(p);
+    }
+    const bool in = p <= half && p <= bound;
+    if (__ballot(in) == 0) {   // the same p in every wave: uniform over the workgroup
+        res = 0;
+        return -1;
+    }
+    const uint64_t q = n - p;
+    bool ok = false;
+    if (in && isp) {
+        if (q <= T.p2_limit) {
+            ok = is_prime_q(q, T);
+        } else if (q < 41) {
+            ok = (0x20A08A28ACull >> q) & 1;   // as in partition_chunk
+        } else if (!small_factor(q)) {
+            ok = w < mr_bases(q) ? mr_test(q, w, mr_wide(q)) : true;
+        }
+    }
+    const uint64_t m = __ballot(ok);
+    if (lane == 0) pass[w] = m;
+    __syncthreads();
+    uint64_t all = ~0ull;
+#pragma unroll
+    for (uint32_t v = 0; v < SERVE_BASES; ++v) all &= pass[v];
+    __syncthreads();   // pass[] is written again by the next chunk
+    if (all) {
+        res = __shfl(p, __ffsll((unsigned long long)all) - 1);
+        return 1;
+    }
+    return 0;
+}
+
 __global__ __launch_bounds__(64 * SERVE_WAVES) void min_partition_kernel(Tables T, const uint64_t* __restrict__ ns,
                                                                          uint32_t count, uint64_t bound,
                                                                          uint64_t* __restrict__ p_out) {
@@ -1575,6 +1622,50 @@ __global__ __launch_bounds__(64 * SERVE_WAVES) void serve_step_kernel(Tables T, 
         }
     }
     if (lane == 0) {
+        out[qi] = ServeDone{res, state, 0};
+        slots[qi] = state ? ServeSlot{0, 0} : ServeSlot{n, base};
+    }
+}
+
+// serve_step_kernel for steps with few occupied slots: one workgroup of SERVE_BASES waves per
+// slot (partition_chunk_wide), so a query's step takes one strong test's time, not two.
+__global__ __launch_bounds__(64 * SERVE_BASES) void serve_step_wide_kernel(Tables T, const uint64_t* __restrict__ adm,
+                                                                           ServeSlot* __restrict__ slots,
+                                                                           uint64_t bound, uint32_t chunks,
+                                                                           ServeDone* __restrict__ out) {
+    __shared__ uint64_t pass[SERVE_BASES];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t qi = blockIdx.x;
+    const uint64_t a = adm[qi];
+    uint64_t n, base;
+    if (a) {
+        n = a;
+        base = 0;
+    } else {
+        const ServeSlot st = slots[qi];
+        n = st.n;
+        base = st.k;
+    }
+    if (n == 0) {   // uniform over the workgroup
+        if (threadIdx.x == 0) out[qi] = ServeDone{0, 2, 0};
+        return;
+    }
+    uint64_t res = 0;
+    uint32_t state = 1;
+    if (n < 4 || (n & 1)) {
+        res = 0;
+    } else if (n == 4) {
+        res = bound >= 2 ? 2 : 0;
+    } else {
+        state = 0;
+        for (uint32_t c = 0; c < chunks; ++c, base += 64) {
+            if (partition_chunk_wide(T, n, n / 2, bound, base, lane, w, pass, res) != 0) {
+                state = 1;
+                break;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
         out[qi] = ServeDone{res, state, 0};
         slots[qi] = state ? ServeSlot{0, 0} : ServeSlot{n, base};
     }
@@ -1755,6 +1846,11 @@ hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, doubl
 hipError_t launch_serve_step(hipStream_t s, const Tables& T, const uint64_t* adm, ServeSlot* slots, uint32_t count,
                              uint64_t bound, uint32_t chunks, ServeDone* out) {
     if (!count) return hipSuccess;
+    if (count <= SERVE_WIDE_MAX) {
+        hipLaunchKernelGGL(serve_step_wide_kernel, dim3(count), dim3(64 * SERVE_BASES), 0, s, T, adm, slots, bound,
+                           chunks, out);
+        return hipGetLastError();
+    }
     const uint32_t blocks = (count + SERVE_WAVES - 1) / SERVE_WAVES;
     hipLaunchKernelGGL(serve_step_kernel, dim3(blocks), dim3(64 * SERVE_WAVES), 0, s, T, adm, slots, count, bound,
                        chunks, out);

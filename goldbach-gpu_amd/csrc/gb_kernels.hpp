@@ -13,6 +13,8 @@ constexpr int SIEVE_BS = 512;    // threads per plain-sieve workgroup
 constexpr int MARK_BS = 512;     // threads per plain-marking workgroup
 constexpr int FUSED_BS = 1024;   // threads per fused persistent workgroup
 constexpr int SERVE_WAVES = 4;   // queries per serving workgroup (one wave each)
+constexpr uint32_t SERVE_BASES = 7;      // strong-test bases of a 64-bit q (the waves of a wide step's workgroup)
+constexpr uint32_t SERVE_WIDE_MAX = 64;  // steps over at most this many slots run one workgroup per slot
 constexpr int NMARK = 1027;      // odd primes in [3, 8193]: the primes the window answers
 
 constexpr uint32_t SIEVE_TILE_MAX = 1u << 18;   // plain sieve: window bits per workgroup, at most
@@ -174,7 +176,9 @@ struct ServeDone {
 };
 
 // One step over slots [0, count): adm[i] != 0 starts query adm[i] in slot i (adm and out are
-// pinned host memory, slots device memory).
+// pinned host memory, slots device memory).  Up to SERVE_WIDE_MAX slots, each slot gets a
+// workgroup whose waves test the candidates to different bases (the latency of one strong test
+// per chunk); above, a wave per slot tests base 2 and confirms the smallest survivor.
 hipError_t launch_serve_step(hipStream_t s, const Tables& T, const uint64_t* adm, ServeSlot* slots, uint32_t count,
                              uint64_t bound, uint32_t chunks, ServeDone* out);
 
