@@ -62,7 +62,7 @@ constexpr uint64_t NA_OFF_SLOW_N = na_align(NA_OFF_OUT + sizeof(StatRec));
 constexpr uint64_t NA_OFF_SLOW_WIN = na_align(NA_OFF_SLOW_N + NA_SLOW_CAP * 8ull);
 constexpr uint64_t NA_OFF_SLOW_RES = na_align(NA_OFF_SLOW_WIN + NA_SLOW_CAP * 4ull);
 constexpr uint64_t NA_OFF_SLOW_CTL = na_align(NA_OFF_SLOW_RES + NA_SLOW_CAP * 8ull);
-constexpr uint64_t NA_SCRATCH_BYTES = na_align(NA_OFF_SLOW_CTL + 16);   // 9.9 MB
+constexpr uint64_t NA_SCRATCH_BYTES = na_align(NA_OFF_SLOW_CTL + 24);   // 9.9 MB
 static_assert(NA_SCRATCH_BYTES < (12ull << 20), "scratch stays modest");
 
 std::mutex g_scratch_mu;
@@ -469,8 +469,8 @@ int enqueue_segment(gb_ctx* c, const SegBufs& B, const SegPlan& sp, uint32_t slo
     if (sp.wwords > B.window_cap || sp.vwords > B.verified_cap || (want_pmin && cnt > B.pmin_cap) ||
         sp.nblocks > B.rec_cap || sp.np > B.r0_cap)
         return fail(c, GB_ERANGE, "segment larger than the buffers it runs on");
-    HCK(c, hipEventRecord(S.ev0, s));
     const bool timed = (c->cfg.flags & GB_CFG_KERNEL_TIMES) != 0;
+    S.seq = ++c->seg_seq;
     S.bits = sp.g.nbits;
     S.primes = sp.np;
     S.evens = cnt;
@@ -485,21 +485,29 @@ int enqueue_segment(gb_ctx* c, const SegBufs& B, const SegPlan& sp, uint32_t slo
     if (timed) HCK(c, hipEventRecord(S.ev_rest, s));
     HCK(c, launch_slow(s, T, mp, B.sl, B.slow_res, B.wins, want_verified ? B.verified : nullptr,
                        want_pmin ? B.pmin : nullptr, A));
-    // the record goes straight to the slot's pinned host memory (no read-back copy)
-    HCK(c, launch_reduce(s, mp, B.rec, B.rec_lo, B.rec_hi, 1, B.sl, B.slow_res, S.h_out));
-    HCK(c, hipEventRecord(S.ev1, s));
-    HCK(c, hipEventRecord(S.done, s));
+    // the record goes straight to the slot's pinned host memory (no read-back copy), then the
+    // segment's device times and its sequence number, which the host waits for
+    HCK(c, launch_reduce(s, mp, B.rec, B.rec_lo, B.rec_hi, 1, B.sl, B.slow_res, S.h_out, S.clk, B.slow_ctl + 2, S.seq));
     return GB_OK;
 }
 
 // Waits for the segment queued in `slot`: its record to *out, its kernel time to *ms.
 int finish_segment(gb_ctx* c, uint32_t slot, uint32_t slow_cap, StatRec* out, double* ms) {
     gb_ctx::SegSlot& S = c->seg_slot[slot];
-    HCK(c, hipEventSynchronize(S.done));
-    float f = 0;
-    HCK(c, hipEventElapsedTime(&f, S.ev0, S.ev1));
+    // spin on the pinned sequence word; now and then ask the stream, so a device error ends the
+    // wait instead of leaving it spinning
+    const volatile uint64_t* seq = &S.clk->seq;
+    for (uint32_t spin = 1; *seq != S.seq; ++spin) {
+        if (spin % 512) continue;
+        const hipError_t e = hipStreamQuery(c->stream);
+        if (e == hipErrorNotReady) continue;
+        HCK(c, e);
+        if (*seq != S.seq) return fail(c, GB_EDEVICE, "plain segment ended without its completion record");
+    }
+    const double f = (double)(S.clk->t1 - S.clk->t0) / c->wall_khz;
     if (ms) *ms = f;
     if (c->cfg.flags & GB_CFG_KERNEL_TIMES) {
+        HCK(c, hipEventSynchronize(S.ev_rest));
         float fs = 0, fm = 0;
         HCK(c, hipEventElapsedTime(&fs, S.ev_sieve, S.ev_mark));
         HCK(c, hipEventElapsedTime(&fm, S.ev_mark, S.ev_rest));
@@ -622,6 +630,12 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
     if (hipGetDeviceProperties(&p, device) != hipSuccess) { c->err = "hipGetDeviceProperties failed"; return bail(GB_EDEVICE); }
     c->cus = p.multiProcessorCount;
     c->total_mem = p.totalGlobalMem;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || wall_khz <= 0) {
+        c->err = "hipDeviceGetAttribute(WallClockRate) failed";
+        return bail(GB_EDEVICE);
+    }
+    c->wall_khz = wall_khz;
     if (c->cfg.phase2_limit > (1ull << 36)) { c->err = "phase2_limit above 2^36"; return bail(GB_EINVAL); }
 
     // device memory guard (validate_hardware_and_limits, goldbach.cu:505-541)
@@ -665,7 +679,7 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
     }
     if (dalloc(c, &c->d_window, window_words) != hipSuccess || dalloc(c, &c->d_verified, verified_words) != hipSuccess ||
         dalloc(c, &c->d_slow_n, SLOW_CAP) != hipSuccess || dalloc(c, &c->d_slow_win, SLOW_CAP) != hipSuccess ||
-        dalloc(c, &c->d_slow_res, SLOW_CAP) != hipSuccess || dalloc(c, &c->d_slow_ctl, 2) != hipSuccess ||
+        dalloc(c, &c->d_slow_res, SLOW_CAP) != hipSuccess || dalloc(c, &c->d_slow_ctl, 3) != hipSuccess ||
         dalloc(c, &c->d_counter, 1) != hipSuccess) {
         c->err = "device allocation failed";
         return bail(GB_ENOMEM);
@@ -679,7 +693,7 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
     {
         // per-slot staging of the plain segments in flight: 256 pinned bytes per slot
         constexpr size_t SLOT = 256;
-        static_assert(sizeof(WinDesc) + sizeof(StatRec) <= SLOT, "slot staging fits");
+        static_assert(sizeof(WinDesc) + sizeof(StatRec) + sizeof(SegClock) <= SLOT, "slot staging fits");
         if (hipHostMalloc(&c->h_slot_block, 2 * SLOT, hipHostMallocDefault) != hipSuccess) {
             c->err = "pinned allocation failed";
             return bail(GB_ENOMEM);
@@ -689,9 +703,9 @@ int gb_open(int device, const gb_config* cfg_in, gb_ctx** out) {
             gb_ctx::SegSlot& S = c->seg_slot[k];
             S.h_win = reinterpret_cast<WinDesc*>(b);
             S.h_out = reinterpret_cast<StatRec*>(b + sizeof(WinDesc));
-            if (hipEventCreate(&S.ev0) != hipSuccess || hipEventCreate(&S.ev1) != hipSuccess ||
-                hipEventCreateWithFlags(&S.done, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreate(&S.ev_sieve) != hipSuccess || hipEventCreate(&S.ev_mark) != hipSuccess ||
+            S.clk = reinterpret_cast<SegClock*>(b + sizeof(WinDesc) + sizeof(StatRec));
+            S.clk->seq = 0;
+            if (hipEventCreate(&S.ev_sieve) != hipSuccess || hipEventCreate(&S.ev_mark) != hipSuccess ||
                 hipEventCreate(&S.ev_rest) != hipSuccess) {
                 c->err = "hipEventCreate failed";
                 return bail(GB_EDEVICE);
@@ -728,7 +742,7 @@ int gb_close(gb_ctx* c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (auto& S : c->seg_slot)
-        for (hipEvent_t e : {S.ev0, S.ev1, S.done, S.ev_sieve, S.ev_mark, S.ev_rest})
+        for (hipEvent_t e : {S.ev_sieve, S.ev_mark, S.ev_rest})
             if (e) (void)hipEventDestroy(e);
     if (c->h_slot_block) dhostfree(c->h_slot_block);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -87,16 +87,20 @@ struct gb_ctx {
     uint64_t* h_lohi = nullptr;        // [rec_lo | rec_hi] of those pieces
     uint64_t* h_ctl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // plain segments in flight, two deep: each slot's pinned descriptor and result record (the
-    // reduction writes it there) and events (kernels start, kernels end, segment done)
+    // plain segments in flight, two deep: each slot's pinned descriptor, result record and
+    // completion (the reduction writes both; the host waits for clk->seq == seq, with no event
+    // between segments: an event record costs the stream about 5 us)
     struct SegSlot {
         gbk::WinDesc* h_win = nullptr;
         gbk::StatRec* h_out = nullptr;
-        hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+        gbk::SegClock* clk = nullptr;
+        uint64_t seq = 0;                                                      // of the queued segment
         hipEvent_t ev_sieve = nullptr, ev_mark = nullptr, ev_rest = nullptr;   // GB_CFG_KERNEL_TIMES
         uint64_t bits = 0, primes = 0, evens = 0;                              // of the queued segment
     };
     SegSlot seg_slot[2];
+    uint64_t seg_seq = 0;        // plain segments queued on this context
+    double wall_khz = 0;         // the device wall clock's rate (wall_clock64 ticks per ms)
     void* h_slot_block = nullptr;
     gb_kernel_times ktimes{};
     // batched launches captured as graphs; a graph's kernels read their pieces from d_wins, so

@@ -1277,7 +1277,9 @@ __global__ void slow_kernel(Tables T, MarkParams mp, SlowList sl, uint64_t* __re
 __global__ __launch_bounds__(256) void reduce_kernel(MarkParams mp, const StatRec* __restrict__ rec,
                                                       const uint64_t* __restrict__ rec_lo,
                                                       const uint64_t* __restrict__ rec_hi, SlowList sl,
-                                                      const uint64_t* __restrict__ res_p, StatRec* __restrict__ out) {
+                                                      const uint64_t* __restrict__ res_p, StatRec* __restrict__ out,
+                                                      SegClock* __restrict__ clk, const uint64_t* __restrict__ t0,
+                                                      uint64_t seq) {
     const uint32_t w = blockIdx.x, tid = threadIdx.x;
     LaneStats s = lane_stats_zero();
     uint64_t evens = 0;
@@ -1334,6 +1336,12 @@ __global__ __launch_bounds__(256) void reduce_kernel(MarkParams mp, const StatRe
         r.evens = ev;
         r.pad = *sl.overflow;   // the slow list overflowed: these statistics are incomplete
         out[w] = r;
+        if (clk) {
+            clk->t0 = *t0;
+            clk->t1 = (uint64_t)wall_clock64();
+            __threadfence_system();   // the record and the times before seq
+            __hip_atomic_store(&clk->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -1366,6 +1374,7 @@ __global__ void segment_prep_kernel(Tables T, WinDesc wd, uint64_t nrec, WinDesc
         rec_hi[0] = nrec;
         slow_ctl[0] = 0;
         slow_ctl[1] = 0;
+        slow_ctl[2] = (uint64_t)wall_clock64();   // the segment's start, for its kernel time
     }
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < wd.np; k += gridDim.x * blockDim.x) {
         const uint32_t p = T.sp[k];
@@ -1802,8 +1811,11 @@ hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, Slo
 }
 
 hipError_t launch_reduce(hipStream_t s, const MarkParams& mp, const StatRec* rec, const uint64_t* rec_lo,
-                         const uint64_t* rec_hi, uint32_t nwin, SlowList sl, const uint64_t* res_p, StatRec* out) {
-    hipLaunchKernelGGL(reduce_kernel, dim3(nwin), dim3(256), 0, s, mp, rec, rec_lo, rec_hi, sl, res_p, out);
+                         const uint64_t* rec_hi, uint32_t nwin, SlowList sl, const uint64_t* res_p, StatRec* out,
+                         SegClock* clk, const uint64_t* t0, uint64_t seq) {
+    if (clk && (nwin != 1 || !t0)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(reduce_kernel, dim3(nwin), dim3(256), 0, s, mp, rec, rec_lo, rec_hi, sl, res_p, out, clk, t0,
+                       seq);
     return hipGetLastError();
 }
 

@@ -139,12 +139,25 @@ hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, Slo
 
 // Merge per-block records of window w in [rec_lo[w], rec_hi[w]) plus its slow entries into out[w]
 // (device or pinned host memory); out[w].pad is nonzero when the slow list overflowed.
+// Completion of a plain segment, written to pinned host memory by its reduction after the result
+// record: the device wall clock (wall_clock64) at the segment's first kernel and at its end, then
+// seq behind a system-scope fence, so a host that reads seq sees the record and both times.
+struct SegClock {
+    uint64_t t0;
+    uint64_t t1;
+    uint64_t seq;
+    uint64_t pad;
+};
+
+// One workgroup per window sums the tile records and the slow list's answers into out[w].  With
+// clk (one window), thread 0 then writes clk->t0 = *t0, clk->t1 and clk->seq = seq.
 hipError_t launch_reduce(hipStream_t s, const MarkParams& mp, const StatRec* rec, const uint64_t* rec_lo,
                          const uint64_t* rec_hi, uint32_t nwin, SlowList sl, const uint64_t* res_p,
-                         StatRec* out);
+                         StatRec* out, SegClock* clk = nullptr, const uint64_t* t0 = nullptr, uint64_t seq = 0);
 
 // One plain segment's set-up: descriptor wins[0] = wd, record bounds [0, nrec), slow-list
 // control cleared, and pr[k] for k < wd.np (the window's sieving-prime entries).
+// slow_ctl: [slow count, overflow, wall clock at the segment's start] (3 words).
 hipError_t launch_segment_prep(hipStream_t s, const Tables& T, const WinDesc& wd, uint64_t nrec, WinDesc* wins,
                                uint64_t* rec_lo, uint64_t* rec_hi, uint64_t* slow_ctl, PrimeRes* pr);
 

@@ -80,7 +80,8 @@ typedef struct gb_seg_stats {
     uint64_t sum_pmin;      /* GB_WANT_PMIN: sum of p_min(n) over n with a partition */
     uint64_t max_pmin;      /* GB_WANT_PMIN: largest p_min(n) */
     uint64_t argmax_n;      /* GB_WANT_PMIN: smallest n with p_min(n) == max_pmin */
-    double   kernel_ms;     /* device time of the call's kernels (HIP events) */
+    double   kernel_ms;     /* device time of the call's kernels (HIP events; plain segments: the device
+                               wall clock from a segment's first kernel to its last) */
 } gb_seg_stats;
 
 /* flags for the verify entry points */
