@@ -1594,45 +1594,70 @@ __global__ __launch_bounds__(64 * SERVE_WAVES) void min_partition_kernel(Tables 
 // device-resident state holds (n = 0: the slot is free).  One wave per slot advances its query
 // by up to `chunks` chunks; out[slot] (pinned host memory) gets the answer and whether it is
 // final, and the state keeps the position of a query that goes on (or frees the slot).
+// A serving step's completion: each workgroup counts itself once its answers are out (its
+// writer fenced them at system scope first); the last one resets the count for the lane's next
+// step and publishes seq to the host, which waits on that pinned word instead of an event.  For
+// the wide step (at most SERVE_WIDE_MAX workgroups, one fence each).
+__device__ __forceinline__ void step_signal(const StepSignal& sg) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t prev = atomicAdd(sg.blocks_done, 1u);
+        if (prev == gridDim.x - 1) {
+            atomicExch(sg.blocks_done, 0u);
+            __threadfence_system();
+            __hip_atomic_store(sg.seq_out, sg.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
+
+// The narrow step's completion, launched behind it on the lane's stream: a system-scope fence per
+// wave of a 16,384-slot step costs more than the step, so the step's answers are ordered before
+// seq by the kernel boundary instead.
+__global__ void serve_signal_kernel(StepSignal sg) {
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(sg.seq_out, sg.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 __global__ __launch_bounds__(64 * SERVE_WAVES) void serve_step_kernel(Tables T, const uint64_t* __restrict__ adm,
                                                                       ServeSlot* __restrict__ slots, uint32_t count,
                                                                       uint64_t bound, uint32_t chunks,
                                                                       ServeDone* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t qi = blockIdx.x * SERVE_WAVES + (threadIdx.x >> 6);
-    if (qi >= count) return;   // uniform per wave
-    const uint64_t a = adm[qi];
-    uint64_t n, base;
-    if (a) {
-        n = a;
-        base = 0;
-    } else {
-        const ServeSlot st = slots[qi];
-        n = st.n;
-        base = st.k;
-    }
-    if (n == 0) {
-        if (lane == 0) out[qi] = ServeDone{0, 2, 0};
-        return;
-    }
-    uint64_t res = 0;
-    uint32_t state = 1;
-    if (n < 4 || (n & 1)) {
-        res = 0;
-    } else if (n == 4) {
-        res = bound >= 2 ? 2 : 0;
-    } else {
-        state = 0;
-        for (uint32_t c = 0; c < chunks; ++c, base += 64) {
-            if (partition_chunk(T, n, n / 2, bound, base, lane, res) != 0) {
-                state = 1;
-                break;
+    if (qi < count) {   // uniform per wave
+        const uint64_t a = adm[qi];
+        uint64_t n, base;
+        if (a) {
+            n = a;
+            base = 0;
+        } else {
+            const ServeSlot st = slots[qi];
+            n = st.n;
+            base = st.k;
+        }
+        uint64_t res = 0;
+        uint32_t state = 1;
+        if (n == 0) {
+            state = 2;   // a free slot
+        } else if (n < 4 || (n & 1)) {
+            res = 0;
+        } else if (n == 4) {
+            res = bound >= 2 ? 2 : 0;
+        } else {
+            state = 0;
+            for (uint32_t c = 0; c < chunks; ++c, base += 64) {
+                if (partition_chunk(T, n, n / 2, bound, base, lane, res) != 0) {
+                    state = 1;
+                    break;
+                }
             }
         }
-    }
-    if (lane == 0) {
-        out[qi] = ServeDone{res, state, 0};
-        slots[qi] = state ? ServeSlot{0, 0} : ServeSlot{n, base};
+        if (lane == 0) {
+            out[qi] = ServeDone{res, state, 0};
+            if (state != 2) slots[qi] = state ? ServeSlot{0, 0} : ServeSlot{n, base};
+        }
     }
 }
 
@@ -1641,7 +1666,8 @@ __global__ __launch_bounds__(64 * SERVE_WAVES) void serve_step_kernel(Tables T, 
 __global__ __launch_bounds__(64 * SERVE_BASES) void serve_step_wide_kernel(Tables T, const uint64_t* __restrict__ adm,
                                                                            ServeSlot* __restrict__ slots,
                                                                            uint64_t bound, uint32_t chunks,
-                                                                           ServeDone* __restrict__ out) {
+                                                                           ServeDone* __restrict__ out,
+                                                                           StepSignal sg) {
     __shared__ uint64_t pass[SERVE_BASES];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t qi = blockIdx.x;
@@ -1655,13 +1681,11 @@ __global__ __launch_bounds__(64 * SERVE_BASES) void serve_step_wide_kernel(Table
         n = st.n;
         base = st.k;
     }
-    if (n == 0) {   // uniform over the workgroup
-        if (threadIdx.x == 0) out[qi] = ServeDone{0, 2, 0};
-        return;
-    }
     uint64_t res = 0;
     uint32_t state = 1;
-    if (n < 4 || (n & 1)) {
+    if (n == 0) {   // a free slot (uniform over the workgroup)
+        state = 2;
+    } else if (n < 4 || (n & 1)) {
         res = 0;
     } else if (n == 4) {
         res = bound >= 2 ? 2 : 0;
@@ -1676,8 +1700,10 @@ __global__ __launch_bounds__(64 * SERVE_BASES) void serve_step_wide_kernel(Table
     }
     if (threadIdx.x == 0) {
         out[qi] = ServeDone{res, state, 0};
-        slots[qi] = state ? ServeSlot{0, 0} : ServeSlot{n, base};
+        if (state != 2) slots[qi] = state ? ServeSlot{0, 0} : ServeSlot{n, base};
+        __threadfence_system();
     }
+    step_signal(sg);
 }
 
 // ---------------------------------------------------------------------------
@@ -1856,16 +1882,18 @@ hipError_t launch_prime_aux(hipStream_t s, const uint32_t* sp, uint32_t n, doubl
 }
 
 hipError_t launch_serve_step(hipStream_t s, const Tables& T, const uint64_t* adm, ServeSlot* slots, uint32_t count,
-                             uint64_t bound, uint32_t chunks, ServeDone* out) {
-    if (!count) return hipSuccess;
+                             uint64_t bound, uint32_t chunks, ServeDone* out, StepSignal sg) {
+    if (!count || !sg.blocks_done || !sg.seq_out) return hipErrorInvalidValue;   // a step must signal
     if (count <= SERVE_WIDE_MAX) {
         hipLaunchKernelGGL(serve_step_wide_kernel, dim3(count), dim3(64 * SERVE_BASES), 0, s, T, adm, slots, bound,
-                           chunks, out);
+                           chunks, out, sg);
         return hipGetLastError();
     }
     const uint32_t blocks = (count + SERVE_WAVES - 1) / SERVE_WAVES;
     hipLaunchKernelGGL(serve_step_kernel, dim3(blocks), dim3(64 * SERVE_WAVES), 0, s, T, adm, slots, count, bound,
                        chunks, out);
+    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+    hipLaunchKernelGGL(serve_signal_kernel, dim3(1), dim3(64), 0, s, sg);
     return hipGetLastError();
 }
 

@@ -192,8 +192,15 @@ struct ServeDone {
 // pinned host memory, slots device memory).  Up to SERVE_WIDE_MAX slots, each slot gets a
 // workgroup whose waves test the candidates to different bases (the latency of one strong test
 // per chunk); above, a wave per slot tests base 2 and confirms the smallest survivor.
+// How a step tells the host it has finished: the workgroups count themselves in *blocks_done
+// (device memory, zero between steps) and the last writes seq to *seq_out (pinned host memory).
+struct StepSignal {
+    uint32_t* blocks_done;
+    uint64_t* seq_out;
+    uint64_t seq;
+};
 hipError_t launch_serve_step(hipStream_t s, const Tables& T, const uint64_t* adm, ServeSlot* slots, uint32_t count,
-                             uint64_t bound, uint32_t chunks, ServeDone* out);
+                             uint64_t bound, uint32_t chunks, ServeDone* out, StepSignal sg);
 
 // Serving: p_out[i] = minimal p for ns[i].
 hipError_t launch_min_partition(hipStream_t s, const Tables& T, const uint64_t* ns, uint32_t count, uint64_t bound,

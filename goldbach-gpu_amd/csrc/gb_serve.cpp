@@ -124,7 +124,9 @@ struct Lane {
     ServeDone* done = nullptr;
     hipStream_t stream = nullptr;   // the lane's own: a step admitted while the other lane's
                                     // runs does not queue behind it
-    hipEvent_t ev = nullptr;        // the lane's step in flight has finished
+    uint32_t* d_blocks = nullptr;   // the step's workgroups that have finished (device memory)
+    uint64_t* h_seq = nullptr;      // pinned: the last step that has finished (written by the step)
+    uint64_t seq = 0;               // the lane's steps launched
     std::vector<SlotInfo> info;
     FreeSlots free_slots{0};
     std::vector<uint32_t> admitted;   // slots whose adm word the step in flight reads
@@ -152,6 +154,7 @@ struct gb_server {
     uint32_t chunks = 0;
     uint64_t queue_cap = 0;
     ServeSlot* d_slots = nullptr;   // query state, device memory (counted against the context's budget)
+    uint32_t* d_blocks = nullptr;   // the lanes' step completion counters (device memory)
     Engine engines[2];              // two engines, each on half of the slots (two lanes each)
     uint64_t engine_resident[2] = {0, 0};
     std::mutex submit_mu;           // one submission at a time, so its ids stay consecutive
@@ -249,10 +252,18 @@ void engine_main(gb_server* s, uint32_t eng) {
         Lane& other = E.lanes[turn ^ 1];
         if (L.size == 0) continue;
         if (L.in_flight) {
-            const hipError_t e = hipEventSynchronize(L.ev);
-            if (e != hipSuccess) {
-                engine_fail(s, std::string("serving engine: ") + hipGetErrorString(e));
-                return;
+            // spin on the step's pinned completion word; now and then ask the stream, so a device
+            // error ends the wait instead of leaving it spinning
+            const volatile uint64_t* done_seq = L.h_seq;
+            for (uint32_t spin = 1; *done_seq != L.seq; ++spin) {
+                if (spin % 512) continue;
+                const hipError_t e = hipStreamQuery(L.stream);
+                if (e == hipErrorNotReady) continue;
+                if (e != hipSuccess || *done_seq != L.seq) {
+                    engine_fail(s, e != hipSuccess ? std::string("serving engine: ") + hipGetErrorString(e)
+                                                   : std::string("serving engine: a step ended without its signal"));
+                    return;
+                }
             }
             L.in_flight = false;
             retire(s, eng, L, results, L.launched_ns);
@@ -309,18 +320,19 @@ void engine_main(gb_server* s, uint32_t eng) {
             s->st.admit_ns += t_a - t_q1;
         }
         if (L.resident == 0) continue;
-        hipError_t e = launch_serve_step(L.stream, s->T, L.adm, L.d_slots, L.hi, s->bound, s->chunks, L.done);
-        if (e == hipSuccess) e = hipEventRecord(L.ev, L.stream);
+        const StepSignal sg{L.d_blocks, L.h_seq, L.seq + 1};
+        const hipError_t e = launch_serve_step(L.stream, s->T, L.adm, L.d_slots, L.hi, s->bound, s->chunks, L.done, sg);
         if (e != hipSuccess) {
             engine_fail(s, std::string("serving engine: ") + hipGetErrorString(e));
             return;
         }
+        L.seq += 1;
         L.in_flight = true;
         L.launched_ns = t_a;
     }
     // stopping: let the steps in flight finish before the pinned words go away
     for (Lane& L : E.lanes)
-        if (L.in_flight) (void)hipEventSynchronize(L.ev);
+        if (L.in_flight) (void)hipStreamSynchronize(L.stream);
 }
 
 void free_server(gb_server* s) {
@@ -331,12 +343,13 @@ void free_server(gb_server* s) {
         for (Lane& L : E.lanes) {
             if (L.adm) (void)hipHostFree(L.adm);
             if (L.done) (void)hipHostFree(L.done);
-            if (L.ev) (void)hipEventDestroy(L.ev);
+            if (L.h_seq) (void)hipHostFree(L.h_seq);
         }
         for (Lane& L : E.lanes)
             if (L.stream) (void)hipStreamDestroy(L.stream);
     }
     if (s->d_slots) dfree(s->ctx, s->d_slots);
+    if (s->d_blocks) dfree(s->ctx, s->d_blocks);
     delete s;
 }
 
@@ -373,6 +386,7 @@ int gb_serve_open(gb_ctx* c, const gb_serve_config* cfg, gb_server** out) {
     s->pool_max = std::clamp<size_t>((64ull << 20) / (((slots + 3) / 4) * sizeof(gb_result)), 4, 1024);
     s->st.slots = s->slots;
     hipError_t e = dalloc(c, &s->d_slots, slots);   // the context's budget counts the slot state
+    if (e == hipSuccess) e = dalloc(c, &s->d_blocks, 4);
     // four lanes of (nearly) equal size, two per engine, laid out in the slot table in order
     uint32_t first = 0;
     for (uint32_t q = 0; q < 4; ++q) {
@@ -381,7 +395,9 @@ int gb_serve_open(gb_ctx* c, const gb_serve_config* cfg, gb_server** out) {
         L.size = (uint32_t)((slots + 3 - q) / 4);
         const size_t n = std::max<uint32_t>(L.size, 1);
         if (e == hipSuccess) e = hipStreamCreate(&L.stream);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&L.ev, hipEventDisableTiming);
+        if (e == hipSuccess) L.d_blocks = s->d_blocks + q;
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_seq), sizeof(uint64_t), hipHostMallocDefault);
+        if (e == hipSuccess) *L.h_seq = 0;
         if (e == hipSuccess) L.d_slots = s->d_slots + first;
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.adm), n * sizeof(uint64_t), hipHostMallocDefault);
         if (e == hipSuccess)
@@ -392,6 +408,7 @@ int gb_serve_open(gb_ctx* c, const gb_serve_config* cfg, gb_server** out) {
         first += L.size;
     }
     if (e == hipSuccess) e = hipMemsetAsync(s->d_slots, 0, slots * sizeof(ServeSlot), s->engines[0].lanes[0].stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_blocks, 0, 4 * sizeof(uint32_t), s->engines[0].lanes[0].stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->engines[0].lanes[0].stream);
     if (e != hipSuccess) {
         const int rc = hip_fail(c, e, __FILE__, __LINE__);
