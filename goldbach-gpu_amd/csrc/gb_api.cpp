@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -118,28 +119,36 @@ void dfree(gb_ctx* c, void* p) {
 }
 
 // Medium-prime schedule for tiles of at most `tb` bits: prime k with
-// p*CHUNK < tb is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK marks (an
-// item skips the multiples of 3, so its CHUNK marks span CHUNK_SPAN*p bits).
+// p*med_div < tb (med_div = 32 by default) is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK
+// marks (an item skips the multiples of 3, so its CHUNK marks span CHUNK_SPAN*p bits); the
+// primes above get a thread each.
 //
 // Wheel form (wheel = true): tb is the bit count of each stream; items are
 // encoded (k << 13) | (stream << 12) | chunk and the streams hold no multiples
-// of 3.  Primes with at least 4 * 64 * WAVE_MARKS marks per stream (p < tb/2048)
-// get wave items: ceil(tb/(64*WAVE_MARKS*p)) per stream, one wavefront each,
+// of 3.  Primes with p*wave_div < tb (wave_div = 2048 by default: at least 4 * 64 * WAVE_MARKS
+// marks per stream) get wave items: ceil(tb/(64*WAVE_MARKS*p)) per stream, one wavefront each,
 // lane l marking l, l + 64, ... of the item's 64*WAVE_MARKS marks; the others
 // get ceil(tb/(CHUNK*p)) lane items of CHUNK consecutive marks per stream.
-int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel) {
+int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel, uint32_t med_div, uint32_t wave_div) {
+    if (!med_div) med_div = LANE_ITEM_DIV;
+    if (!wave_div) wave_div = WAVE_ITEM_DIV;
     for (auto& it : c->items)
-        if (it.tb == tb && it.wheel == (wheel ? 1u : 0u)) { *out = it; return GB_OK; }
+        if (it.tb == tb && it.wheel == (wheel ? 1u : 0u) && it.med_div == med_div && it.wave_div == wave_div) {
+            *out = it;
+            return GB_OK;
+        }
     ItemTable t;
     t.tb = tb;
     t.wheel = wheel ? 1u : 0u;
+    t.med_div = med_div;
+    t.wave_div = wave_div;
     std::vector<uint32_t> items, items_p, start;
     uint32_t k = NPAT;
     start.assign(NPAT + 1, 0);
     t.k_wave = NPAT;
-    while (k < c->nsp && (uint64_t)c->h_sp[k] * CHUNK < tb) {
+    while (k < c->nsp && (uint64_t)c->h_sp[k] * med_div < tb) {
         const uint32_t p = c->h_sp[k];
-        const bool wave_item = wheel && (uint64_t)p * 4 * 64 * WAVE_MARKS < tb;
+        const bool wave_item = wheel && (uint64_t)p * wave_div < tb;
         if (wave_item) t.k_wave = k + 1;
         const uint64_t span = (uint64_t)(wave_item ? 64 * WAVE_MARKS : (wheel ? CHUNK : CHUNK_SPAN)) * p;
         const uint32_t chunks = (uint32_t)((tb + span - 1) / span);
@@ -876,6 +885,8 @@ struct Shape {
     bool wheel;
     uint32_t tile_log2;   // fused: tiles of 2^tile_log2 evens; wheel: 3 * 2^(tile_log2 - 1) evens
     uint32_t wg;          // persistent workgroups per CU
+    uint32_t med_div = 0;    // work-item bounds of the medium-prime schedule (0 = defaults)
+    uint32_t wave_div = 0;
 };
 
 // Even n [start, start + 2(evens - 1)] that belong to output `owner`.
@@ -886,10 +897,11 @@ struct Piece {
 
 // Tile shape of a kind: the defaults when neither knob is given, else the knobs (one workgroup per
 // CU and 2^20-even tiles where only one of them is).
-Shape shape_of(bool wheel, uint32_t tile_log2, uint32_t wg) {
+Shape shape_of(bool wheel, uint32_t tile_log2, uint32_t wg, uint32_t med_div = 0, uint32_t wave_div = 0) {
     if (tile_log2 == 0 && wg == 0)
-        return wheel ? Shape{true, WHEEL_TILE_LOG2, 2} : Shape{false, DEFAULT_TILE_LOG2, 1};
-    return Shape{wheel, tile_log2 ? tile_log2 : DEFAULT_TILE_LOG2, wg ? wg : 1};
+        return wheel ? Shape{true, WHEEL_TILE_LOG2, 2, med_div, wave_div}
+                     : Shape{false, DEFAULT_TILE_LOG2, 1, med_div, wave_div};
+    return Shape{wheel, tile_log2 ? tile_log2 : DEFAULT_TILE_LOG2, wg ? wg : 1, med_div, wave_div};
 }
 
 int check_shape(gb_ctx* c, const Shape& sh) {
@@ -897,6 +909,9 @@ int check_shape(gb_ctx* c, const Shape& sh) {
     if (sh.wg > 16) return fail(c, GB_EINVAL, "wg_per_cu must be at most 16");
     if (sh.wheel && sh.wg >= 2 && (1u << (sh.tile_log2 - 1)) > WH_CLASS_HALF)
         return fail(c, GB_EINVAL, "two workgroups per CU take tiles of at most 2^19 evens");
+    if (sh.med_div && (sh.med_div < 8 || sh.med_div > 256)) return fail(c, GB_EINVAL, "lane_item_div must be in [8, 256]");
+    if (sh.wave_div && (sh.wave_div < 512 || sh.wave_div > 65536))
+        return fail(c, GB_EINVAL, "wave_item_div must be in [512, 65536]");
     return GB_OK;
 }
 
@@ -945,8 +960,12 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
     const MarkParams mp = mark_params(c, p_small, flags);
     const uint32_t cls_len = 1u << (sh.tile_log2 - 1), te = 1u << sh.tile_log2;
     ItemTable it;
-    int rc = sh.wheel ? build_items(c, cls_len + WH_HALO + 64, &it, true) : build_items(c, te + HALO + WIN_PAD, &it);
+    int rc = sh.wheel ? build_items(c, cls_len + WH_HALO + 64, &it, true, sh.med_div, sh.wave_div)
+                      : build_items(c, te + HALO + WIN_PAD, &it, false, sh.med_div);
     if (rc) return rc;
+    const size_t lds = sh.wheel ? wheel_lds_bytes(it.n_med, sh.wg) : fused_lds_bytes(te, it.n_med);
+    if (lds > LDS_PER_CU)
+        return fail(c, GB_EINVAL, "the work items of this lane_item_div need more LDS than a CU holds at this tile size");
     const Tables T = make_tables(c, it);
     rc = ensure_wins(c, np);
     if (rc) return rc;
@@ -989,7 +1008,8 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
     uint64_t grid = (uint64_t)c->cus * sh.wg;
     if (grid > tiles) grid = tiles;
     SlowList sl = slow_list(c);
-    const std::vector<uint64_t> key = {sh.wheel ? 1ull : 0ull, sh.tile_log2, sh.wg, np, tiles, grid, p_small, flags,
+    const std::vector<uint64_t> key = {sh.wheel ? 1ull : 0ull, sh.tile_log2, sh.wg, sh.med_div, sh.wave_div,
+                                       np, tiles, grid, p_small, flags,
                                        (uint64_t)(uintptr_t)c->d_rec, (uint64_t)(uintptr_t)c->d_tinfo,
                                        (uint64_t)(uintptr_t)c->d_wins,
                                        (uint64_t)(uintptr_t)c->d_out, (uint64_t)(uintptr_t)c->d_r0,
@@ -1116,9 +1136,11 @@ int plan_batched(gb_ctx* c, uint64_t start, uint64_t limit, const gb_tuning& t, 
     for (size_t k = 0; k < kinds.size(); ++k) {
         Part P;
         const bool wheel = kinds[k].first;
-        if (auto_pick && k == dom) P.sh = shape_of(wheel, auto_pick->tile_log2, auto_pick->wg_per_cu);
+        if (auto_pick && k == dom)
+            P.sh = shape_of(wheel, auto_pick->tile_log2, auto_pick->wg_per_cu, auto_pick->lane_item_div,
+                            auto_pick->wave_item_div);
         else if (auto_pick) P.sh = shape_of(wheel, 0, 0);
-        else P.sh = shape_of(wheel, t.tile_log2, t.wg_per_cu);
+        else P.sh = shape_of(wheel, t.tile_log2, t.wg_per_cu, t.lane_item_div, t.wave_item_div);
         int rc = check_shape(c, P.sh);
         if (rc) return rc;
         cut_pieces(P.sh, kinds[k].second.first, kinds[k].second.second, 0, CHUNK_TILES, P.pieces);
@@ -1262,34 +1284,64 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
     const MarkParams mp = mark_params(c, p_small, flags);
     double best_s = 0;
     gb_tuning pick = def;
-    for (const gb_tuning& t : cand) {
-        double secs = 1e30;
-        for (int rep = 0; rep < 3; ++rep) {   // the first run of a candidate builds its schedule and graph
-            const auto t0 = std::chrono::steady_clock::now();
-            if (form == GB_FORM_BATCHED) {
-                const Shape sh{wheel, t.tile_log2, t.wg_per_cu};
-                std::vector<Piece> pcs;
-                for (const WinDesc& w : sample) cut_pieces(sh, w.start, w.start + 2 * (w.evens - 1), 0, CHUNK_TILES, pcs);
-                gb_seg_stats sink{};
-                double ms = 0;
-                bool failed = false;
-                rc = run_pieces(c, sh, pcs, p_small, flags, &sink, false, nullptr, nullptr, &ms, &failed);
-                if (rc) return rc;
-            } else {
-                for (const WinDesc& w : sample) {
-                    gb_seg_stats tot{};
-                    rc = run_plain(c, w.start, w.start + 2 * (w.evens - 1), t.seg_evens, mp, &tot, nullptr);
-                    if (rc) return rc;
+    // each candidate runs once to build its schedule and graph, then twice timed; the fastest wins
+    auto time_candidates = [&](const std::vector<gb_tuning>& cands) -> int {
+        for (const gb_tuning& t : cands) {
+            double secs = 1e30;
+            for (int rep = 0; rep < 3; ++rep) {
+                const auto t0 = std::chrono::steady_clock::now();
+                if (form == GB_FORM_BATCHED) {
+                    const Shape sh{wheel, t.tile_log2, t.wg_per_cu, t.lane_item_div, t.wave_item_div};
+                    std::vector<Piece> pcs;
+                    for (const WinDesc& w : sample)
+                        cut_pieces(sh, w.start, w.start + 2 * (w.evens - 1), 0, CHUNK_TILES, pcs);
+                    gb_seg_stats sink{};
+                    double ms = 0;
+                    bool failed = false;
+                    const int r = run_pieces(c, sh, pcs, p_small, flags, &sink, false, nullptr, nullptr, &ms, &failed);
+                    if (r) return r;
+                } else {
+                    for (const WinDesc& w : sample) {
+                        gb_seg_stats tot{};
+                        const int r = run_plain(c, w.start, w.start + 2 * (w.evens - 1), t.seg_evens, mp, &tot, nullptr);
+                        if (r) return r;
+                    }
                 }
+                const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (rep > 0) secs = std::min(secs, dt);
             }
-            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (rep > 0) secs = std::min(secs, dt);
+            const double rate = 4.0 * (double)win / secs;
+            if (rate > best_s) {
+                best_s = rate;
+                pick = t;
+            }
         }
-        const double rate = 4.0 * (double)win / secs;
-        if (rate > best_s) {
-            best_s = rate;
-            pick = t;
-        }
+        return GB_OK;
+    };
+    rc = time_candidates(cand);
+    if (rc) return rc;
+    if (form == GB_FORM_BATCHED) {
+        // second round: the medium primes' work-item bounds on the fastest tile shape
+        const gb_tuning shape = pick;
+        const uint32_t tl = shape.tile_log2;
+        const uint32_t tb = wheel ? (1u << (tl - 1)) + WH_HALO + 64 : (1u << tl) + HALO + WIN_PAD;
+        const std::vector<uint32_t> waves = wheel ? std::vector<uint32_t>{1024u, WAVE_ITEM_DIV} : std::vector<uint32_t>{0u};
+        std::vector<gb_tuning> items;
+        for (uint32_t md : {24u, LANE_ITEM_DIV, 48u})
+            for (uint32_t wd : waves) {
+                if (md == LANE_ITEM_DIV && (wd == WAVE_ITEM_DIV || wd == 0)) continue;   // the first round's
+                ItemTable it;
+                rc = build_items(c, tb, &it, wheel, md, wd);
+                if (rc) return rc;
+                const size_t lds = wheel ? wheel_lds_bytes(it.n_med, shape.wg_per_cu) : fused_lds_bytes(1u << tl, it.n_med);
+                if (lds * shape.wg_per_cu > LDS_PER_CU) continue;
+                gb_tuning t = shape;
+                t.lane_item_div = md;
+                t.wave_item_div = wd;
+                items.push_back(t);
+            }
+        rc = time_candidates(items);
+        if (rc) return rc;
     }
     c->tune_cache.emplace_back(key, pick);
     *best = pick;

@@ -17,12 +17,16 @@
 namespace gbh {
 
 constexpr uint32_t PLAIN_TILE_BITS = 1u << 18;   // plain sieve tile (bits per workgroup)
+constexpr uint32_t LANE_ITEM_DIV = 32;            // default: lane items for p * 32 < tile bits
+constexpr uint32_t WAVE_ITEM_DIV = 2048;          // default: wave items for p * 2048 < stream bits
 
 struct ItemTable {
     uint32_t tb = 0;        // tile bits the schedule was cut for
     uint32_t wheel = 0;     // 1: items per (prime, stream) of the mod-6 wheel
     uint32_t n_med = 0;
     uint32_t k_wave = 0;    // wheel: primes below index k_wave are cut into wave items
+    uint32_t med_div = 0;   // lane items for p * med_div < tb
+    uint32_t wave_div = 0;  // wheel: wave items for p * wave_div < tb
     uint32_t* d_items = nullptr;
     uint32_t* d_items_p = nullptr;
     uint32_t* d_start = nullptr;
@@ -158,7 +162,9 @@ hipError_t dalloc(gb_ctx* c, T** p, uint64_t count) {
 void dfree(gb_ctx* c, void* p);
 
 // Medium-prime schedule of the sieve for tiles of at most `tb` bits (cached on c).
-int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel = false);
+// med_div / wave_div: the work-item bounds (0 = LANE_ITEM_DIV / WAVE_ITEM_DIV).
+int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel = false, uint32_t med_div = 0,
+                uint32_t wave_div = 0);
 // Device table view of c with the schedule `it`.
 gbk::Tables make_tables(const gb_ctx* c, const ItemTable& it);
 

@@ -163,6 +163,11 @@ typedef struct gb_tuning {
     uint64_t seg_evens;     /* plain: evens per segment (0 = 2^27) */
     uint32_t wg_per_cu;     /* batched: persistent workgroups per CU (0 = default; 2 needs tile_log2 <= 19) */
     uint32_t flags;         /* GB_WANT_PMIN | GB_TUNE_AUTO */
+    uint32_t lane_item_div; /* batched: primes p with p * lane_item_div below a tile's stream bits are
+                               cut into work items of 32 consecutive marks, larger ones get a thread
+                               each (0 = 32; 8..256) */
+    uint32_t wave_item_div; /* batched wheel: primes p with p * wave_item_div below the stream bits
+                               are cut into wavefront items of 512 marks (0 = 2048; 512..65536) */
 } gb_tuning;
 
 #define GB_TUNE_AUTO 0x2u   /* gb_tuning.flags: take tile / segment size from gb_autotune (cached per context) */

@@ -148,6 +148,7 @@ def test_autotune_picks_a_valid_tuning(ctx, gb):
     want = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_PLAIN, seg_evens=1 << 27)
     t, rate = ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED)
     assert 16 <= t["tile_log2"] <= 20 and t["wg_per_cu"] in (1, 2) and rate > 1e10
+    assert t["lane_item_div"] in (0, 24, 32, 48) and t["wave_item_div"] in (0, 1024, 2048)
     assert ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED)[0] == t   # cached
     got = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED, autotune=True)
     assert (got["evens"], got["phase2"], got["failures"]) == (want["evens"], want["phase2"], want["failures"])
@@ -221,3 +222,22 @@ def test_range_sums_equal_fixture(ctx, gb, form):
         for k in ("evens", "phase2", "failures", "first_fail_n", "sum_pmin", "max_pmin", "argmax_n"):
             assert st[k] == r[k], (r["start"], r["limit"], k)
 
+
+@pytest.mark.parametrize("lane_div,wave_div", [(24, 1024), (48, 2048), (16, 4096)])
+def test_work_item_bounds_leave_the_answers_alone(ctx, gb, lane_div, wave_div):
+    # the medium primes' work-item bounds (which primes are cut into lane items, which into wave
+    # items, which get a thread each) move work between the sieve's code paths; the statistics of
+    # [4, 1e10] and of the window near 1e12 must not move
+    for r in _range_fixtures():
+        if r["limit"] - r["start"] > 2 * 10**10:
+            continue
+        st = ctx.verify_range(r["start"], r["limit"], r["p_small"], form=gb.GB_FORM_BATCHED, want_pmin=True,
+                              lane_item_div=lane_div, wave_item_div=wave_div)
+        for k in ("evens", "phase2", "failures", "first_fail_n", "sum_pmin", "max_pmin", "argmax_n"):
+            assert st[k] == r[k], (r["start"], r["limit"], lane_div, wave_div, k)
+
+
+def test_work_item_bounds_are_checked(ctx, gb):
+    for kw in ({"lane_item_div": 4}, {"lane_item_div": 512}, {"wave_item_div": 100}):
+        with pytest.raises(gb.GoldbachError):
+            ctx.verify_range(4, 10**8, 10**6, form=gb.GB_FORM_BATCHED, **kw)
