@@ -119,7 +119,7 @@ void dfree(gb_ctx* c, void* p) {
 }
 
 // Medium-prime schedule for tiles of at most `tb` bits: prime k with
-// p*med_div < tb (med_div = 32 by default) is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK
+// p*med_div < tb (med_div = 32 by default, 96 in the wheel) is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK
 // marks (an item skips the multiples of 3, so its CHUNK marks span CHUNK_SPAN*p bits); the
 // primes above get a thread each.
 //
@@ -130,7 +130,7 @@ void dfree(gb_ctx* c, void* p) {
 // lane l marking l, l + 64, ... of the item's 64*WAVE_MARKS marks; the others
 // get ceil(tb/(CHUNK*p)) lane items of CHUNK consecutive marks per stream.
 int build_items(gb_ctx* c, uint32_t tb, ItemTable* out, bool wheel, uint32_t med_div, uint32_t wave_div) {
-    if (!med_div) med_div = LANE_ITEM_DIV;
+    if (!med_div) med_div = wheel ? WHEEL_LANE_ITEM_DIV : LANE_ITEM_DIV;
     if (!wave_div) wave_div = WAVE_ITEM_DIV;
     for (auto& it : c->items)
         if (it.tb == tb && it.wheel == (wheel ? 1u : 0u) && it.med_div == med_div && it.wave_div == wave_div) {
@@ -1327,9 +1327,10 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
         const uint32_t tb = wheel ? (1u << (tl - 1)) + WH_HALO + 64 : (1u << tl) + HALO + WIN_PAD;
         const std::vector<uint32_t> waves = wheel ? std::vector<uint32_t>{1024u, WAVE_ITEM_DIV} : std::vector<uint32_t>{0u};
         std::vector<gb_tuning> items;
-        for (uint32_t md : {24u, LANE_ITEM_DIV, 48u})
+        const uint32_t md0 = wheel ? WHEEL_LANE_ITEM_DIV : LANE_ITEM_DIV;
+        for (uint32_t md : {md0 / 2, md0, md0 * 2})
             for (uint32_t wd : waves) {
-                if (md == LANE_ITEM_DIV && (wd == WAVE_ITEM_DIV || wd == 0)) continue;   // the first round's
+                if (md == md0 && (wd == WAVE_ITEM_DIV || wd == 0)) continue;   // the first round's
                 ItemTable it;
                 rc = build_items(c, tb, &it, wheel, md, wd);
                 if (rc) return rc;

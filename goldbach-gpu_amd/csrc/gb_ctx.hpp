@@ -17,7 +17,8 @@
 namespace gbh {
 
 constexpr uint32_t PLAIN_TILE_BITS = 1u << 18;   // plain sieve tile (bits per workgroup)
-constexpr uint32_t LANE_ITEM_DIV = 32;            // default: lane items for p * 32 < tile bits
+constexpr uint32_t LANE_ITEM_DIV = 32;            // default: lane items for p * 32 < tile bits (odd-only tiles)
+constexpr uint32_t WHEEL_LANE_ITEM_DIV = 96;      // ... and p * 96 < stream bits in the wheel
 constexpr uint32_t WAVE_ITEM_DIV = 2048;          // default: wave items for p * 2048 < stream bits
 
 struct ItemTable {

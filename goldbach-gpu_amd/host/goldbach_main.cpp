@@ -271,7 +271,7 @@ int main(int argc, char** argv) {
         if (form == GB_FORM_BATCHED)
             std::cout << "Autotuned      : tile 2^" << tun.tile_log2 << " evens, " << tun.wg_per_cu
                       << " workgroup(s) per CU, lane items below stream/"
-                      << (tun.lane_item_div ? tun.lane_item_div : 32u) << ", wave items below stream/"
+                      << (tun.lane_item_div ? tun.lane_item_div : 96u) << ", wave items below stream/"
                       << (tun.wave_item_div ? tun.wave_item_div : 2048u);
         else
             std::cout << "Autotuned      : segments of " << tun.seg_evens << " evens";

@@ -165,7 +165,7 @@ typedef struct gb_tuning {
     uint32_t flags;         /* GB_WANT_PMIN | GB_TUNE_AUTO */
     uint32_t lane_item_div; /* batched: primes p with p * lane_item_div below a tile's stream bits are
                                cut into work items of 32 consecutive marks, larger ones get a thread
-                               each (0 = 32; 8..256) */
+                               each (0 = 96 in the wheel, 32 in odd-only tiles; 8..256) */
     uint32_t wave_item_div; /* batched wheel: primes p with p * wave_item_div below the stream bits
                                are cut into wavefront items of 512 marks (0 = 2048; 512..65536) */
 } gb_tuning;
