@@ -89,6 +89,144 @@ PROBE_LIB = os.path.join(ROOT, "goldbach-gpu_amd", "lib", "libgb_probe.so")
 PROFILED_SOURCES = [os.path.join(ROOT, "goldbach-gpu_amd", "csrc", f)
                     for f in ("gb_kernels.hip", "gb_device.hpp", "gb_kernels.hpp", "gb_api.cpp", "gb_ctx.hpp")]
 
+# The rocprofv3 runs behind the committed counter profiles: name -> (rocprofv3 options, program).
+# `bench.py --profile-commands DIR` prints them (each under its own time limit), and
+# `bench.py --summarize-profiles DIR` rebuilds profiles/*.json from their CSV output.
+HEADLINE_CMD = ("python3 -u bench.py --steps 1 --warmup 0 --no-cpu --no-ref-cpu --no-plain --no-serve --no-windows "
+                "--no-counts --no-big")
+WINDOWS_CMD = "python3 -u bench.py --steps 1 --warmup 0 --no-cpu --no-ref-cpu --no-plain --no-serve --no-counts --no-big"
+PROFILE_RUNS = {
+    "t": ("--kernel-trace --stats", HEADLINE_CMD),
+    "s": ("--pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES SQ_LDS_BANK_CONFLICT SQ_LDS_IDX_ACTIVE "
+          "SQ_BUSY_CYCLES SQ_WAVE_CYCLES", HEADLINE_CMD),
+    "a": ("--pmc SQ_WAVE_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY SQC_ICACHE_MISSES", HEADLINE_CMD),
+    "f": ("--pmc FETCH_SIZE", HEADLINE_CMD),
+    "w": ("--pmc WRITE_SIZE", HEADLINE_CMD),
+    "cf": ("--pmc FETCH_SIZE", "python3 -u bench.py --copy-probe"),
+    "cw": ("--pmc WRITE_SIZE", "python3 -u bench.py --copy-probe"),
+    "ht": ("--kernel-trace --stats", WINDOWS_CMD),
+    "hs": ("--pmc SQ_INSTS_VALU SQ_INSTS_VALU_INT32 SQ_INSTS_VALU_INT64 SQ_INSTS_SALU SQ_INSTS_LDS SQ_WAVES "
+           "SQ_BUSY_CYCLES", WINDOWS_CMD),
+    "p": ("--pmc SQ_INSTS_VALU SQ_BUSY_CYCLES SQ_WAVES", "python3 -u bench.py --valu-probe"),
+}
+COPY_PROBE_BYTES = 1 << 30
+
+
+def profile_command(name: str, out_dir: str) -> str:
+    opts, prog = PROFILE_RUNS[name]
+    return f"rocprofv3 {opts} -f csv -d {out_dir}/{name} -o {name} -- {prog}"
+
+
+def _csv_rows(out_dir: str, name: str, suffix: str) -> list:
+    import csv
+    import glob
+
+    rows = []
+    for path in sorted(glob.glob(os.path.join(out_dir, name, "**", f"*{suffix}.csv"), recursive=True)):
+        with open(path, newline="") as f:
+            rows.extend(csv.DictReader(f))
+    if not rows:
+        raise SystemExit(f"no {suffix}.csv under {os.path.join(out_dir, name)}")
+    return rows
+
+
+def _dispatches(out_dir: str, name: str) -> list:
+    """Counter values per kernel dispatch, in dispatch order: [(kernel name, {counter: value}, ns)]."""
+    by = {}
+    for r in _csv_rows(out_dir, name, "counter_collection"):
+        d = by.setdefault(int(r["Dispatch_Id"]), [r["Kernel_Name"], {}, 0])
+        d[1][r["Counter_Name"]] = d[1].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        d[2] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    return [tuple(by[k]) for k in sorted(by)]
+
+
+def _kernel_stats(out_dir: str, name: str, key: str) -> dict:
+    for r in _csv_rows(out_dir, name, "kernel_stats"):
+        if key in r["Name"]:
+            return {"calls": int(r["Calls"]), "total_ms": float(r["TotalDurationNs"]) * 1e-6,
+                    "avg_ms": float(r["AverageNs"]) * 1e-6}
+    raise SystemExit(f"{key} not in the kernel stats of {name}")
+
+
+def summarize_profiles(out_dir: str) -> int:
+    """Rebuild profiles/wheel_kernel_1e12_traffic.json and profiles/high_windows_mr.json from the
+    CSV output of the PROFILE_RUNS under out_dir, stamped with the current build hash."""
+    wheel, fused = "wheel_kernel<512", "fused_kernel<1024>"
+
+    def summed(name, key):
+        tot = {}
+        for kname, ctr, _ in _dispatches(out_dir, name):
+            if key in kname:
+                for c, v in ctr.items():
+                    tot[c] = tot.get(c, 0.0) + v
+        return tot
+
+    def copy_kib(name, counter):
+        vals = [ctr[counter] for kname, ctr, _ in _dispatches(out_dir, name) if "copyBuffer" in kname]
+        return max(vals)
+
+    cf_kib, cw_kib = copy_kib("cf", "FETCH_SIZE"), copy_kib("cw", "WRITE_SIZE")
+    fetch_scale = round(COPY_PROBE_BYTES / (cf_kib * 1024.0), 3)
+    write_scale = round(COPY_PROBE_BYTES / (cw_kib * 1024.0), 3)
+    trace = _kernel_stats(out_dir, "t", wheel)
+    s, a = summed("s", wheel), summed("a", wheel)
+    fetch, write = summed("f", wheel)["FETCH_SIZE"], summed("w", wheel)["WRITE_SIZE"]
+    hbm = fetch * 1024 * fetch_scale + write * 1024 * write_scale
+    prof = {
+        "kernel": "gbk::wheel_kernel<512, 262144u>",
+        "workload": "all even n in [4, 1e12], batched persistent form, mod-6 wheel tiles of 3 x 2^18 evens, two "
+                    "workgroups per CU, P_SMALL = 1e6: one pass = %d launches of at most 2^17 tiles" % trace["calls"],
+        "build_hash": build_hash(),
+        "launches_per_pass": trace["calls"],
+        "trace_ms_per_pass": round(trace["total_ms"], 3),
+        "trace_ms_per_launch": round(trace["avg_ms"], 3),
+        "fetch_size_kib_per_pass": fetch, "fetch_scale": fetch_scale,
+        "write_size_kib_per_pass": write, "write_scale": write_scale,
+        "hbm_bytes_per_pass": int(hbm),
+        "sq_insts_valu_per_pass": s["SQ_INSTS_VALU"], "sq_insts_lds_per_pass": s["SQ_INSTS_LDS"],
+        "sq_insts_salu_per_pass": s["SQ_INSTS_SALU"], "sq_lds_bank_conflict": s["SQ_LDS_BANK_CONFLICT"],
+        "sq_lds_idx_active": s["SQ_LDS_IDX_ACTIVE"], "sq_busy_cycles": s["SQ_BUSY_CYCLES"],
+        "sq_wave_cycles": s["SQ_WAVE_CYCLES"], "sq_waves": s["SQ_WAVES"],
+        "wait": {k: a[k] for k in sorted(a)},
+        "calibration": "1 GiB device-to-device copy (bench.py --copy-probe): FETCH_SIZE %.0f KiB, WRITE_SIZE %.0f KiB"
+                       % (cf_kib, cw_kib),
+        "commands": [profile_command(n, "<dir>") for n in ("t", "s", "a", "f", "w", "cf", "cw")],
+        "note": "counter values summed over the pass's wheel_kernel dispatches (rocprofv3 CSV output, "
+                "counter_collection rows per dispatch)",
+    }
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(prof, f, indent=1)
+        f.write("\n")
+    runs = [(ctr, ns) for kname, ctr, ns in _dispatches(out_dir, "hs") if fused in kname]
+    hs_all = summed("hs", fused)
+    htrace = _kernel_stats(out_dir, "ht", fused)
+    main_ctr, main_ns = max(runs, key=lambda x: x[0]["SQ_INSTS_VALU"])   # the measured launch (the other: warm-up)
+    hprof = {
+        "kernel": "gbk::fused_kernel<1024>",
+        "workload": "64 windows of uniform 1e6..1e9 evens at even starts log-uniform in [1e15, 1.8e19] (numpy seed "
+                    "0): pre-sieve with every odd prime below 2^22, base-2 strong test, strong Lucas test",
+        "build_hash": build_hash(),
+        "sq_insts_valu_per_launch": main_ctr["SQ_INSTS_VALU"],
+        "sq_insts_valu_int64_per_launch": main_ctr["SQ_INSTS_VALU_INT64"],
+        "sq_insts_valu_int32_per_launch": main_ctr["SQ_INSTS_VALU_INT32"],
+        "kernel_ms_under_profiler": round(main_ns * 1e-6, 3),
+        "trace_launches": htrace["calls"], "trace_ms_all_launches": round(htrace["total_ms"], 3),
+        "sq_insts_valu_int64_launches": hs_all["SQ_INSTS_VALU_INT64"],
+        "sq_insts_valu_int32_launches": hs_all["SQ_INSTS_VALU_INT32"],
+        "sq_insts_salu_launches": hs_all["SQ_INSTS_SALU"], "sq_insts_lds_launches": hs_all["SQ_INSTS_LDS"],
+        "commands": [profile_command(n, "<dir>") for n in ("ht", "hs")],
+        "note": "the *_per_launch figures are the measured launch (the larger fused_kernel dispatch; the other is "
+                "the warm-up on the same windows cut to 2^20 evens); the *_launches figures sum both",
+    }
+    with open(WINDOWS_PROFILE, "w") as f:
+        json.dump(hprof, f, indent=1)
+        f.write("\n")
+    probe = [(kname[kname.index("valu_probe"):kname.index(">") + 1], ctr, ns)
+             for kname, ctr, ns in _dispatches(out_dir, "p") if "valu_probe" in kname]
+    print(json.dumps({"wheel": prof, "windows": hprof, "windows_trace": htrace, "copy_kib": [cf_kib, cw_kib],
+                      "probe": [{"kernel": k, "ns": ns, **c} for k, c, ns in probe]}, indent=1))
+    return 0
+
 
 def build_hash() -> str:
     """16 hex digits of SHA-256 over the profiled sources; committed profiles carry the value
@@ -454,6 +592,10 @@ def windows_leg(ctx, probe) -> dict:
     prof = load_profile(WINDOWS_PROFILE)
     peak = probe["v_add_u32_independent"]["wave_insts_per_s"] if probe else None
     achieved = prof["sq_insts_valu_per_launch"] / ksec if prof else None
+    # 64-bit integer instructions (v_mad_u64_u32, 64-bit adds) issue at half the v_add_u32 rate:
+    # counted as two issue slots each, the launch's slots over the live time against the same peak
+    int64 = prof.get("sq_insts_valu_int64_per_launch") if prof else None
+    slots = (prof["sq_insts_valu_per_launch"] + int64) / ksec if int64 else None
     return {"workload": "64 windows, uniform 1e6..1e9 evens, starts log-uniform in [1e15, 1.8e19] (seed 0)",
             "evens": evens, "value": evens / dt, "unit": "evens/s", "seconds": dt,
             "kernel_ms": tot["kernel_ms"], "phase2": tot["phase2"],
@@ -463,6 +605,9 @@ def windows_leg(ctx, probe) -> dict:
                          "counted": "SQ_INSTS_VALU of the launch, profiles/high_windows_mr.json (null when that "
                                     "profile was taken with other sources)",
                          "peak_note": "v_add_u32 issue rate of gb_probe_valu, measured in this run",
+                         "issue_slots_per_s": slots,
+                         "issue_slots_frac": slots / peak if (slots and peak) else None,
+                         "issue_slots_note": "each 64-bit VALU instruction (SQ_INSTS_VALU_INT64) counted twice",
                          "montgomery_products_model": products,
                          "montgomery_products_per_s": products / ksec}}
 
@@ -568,7 +713,17 @@ def main() -> int:
                     help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the VALU issue probes (for rocprofv3 SQ_INSTS_VALU / SQ_BUSY_CYCLES)")
+    ap.add_argument("--profile-commands", metavar="DIR",
+                    help="print the rocprofv3 runs behind profiles/*.json, output under DIR, and exit")
+    ap.add_argument("--summarize-profiles", metavar="DIR",
+                    help="rebuild profiles/*.json from the CSV output of those runs under DIR, and exit")
     args = ap.parse_args()
+    if args.profile_commands:
+        for name in PROFILE_RUNS:
+            print(profile_command(name, args.profile_commands))
+        return 0
+    if args.summarize_profiles:
+        return summarize_profiles(args.summarize_profiles)
     if args.valu_probe:
         print(json.dumps({"valu_probe": valu_probe(), "build_hash": build_hash()}))
         return 0
@@ -607,9 +762,11 @@ def main() -> int:
     probe = valu_probe()
     peak_valu = probe["v_add_u32_independent"]["wave_insts_per_s"]
     prof = load_profile(TRAFFIC_FILE)         # counters of the current sources, or None
-    traffic = prof.get("hbm_bytes_per_pass") if prof else None
+    traffic_pass = prof.get("hbm_bytes_per_pass") if prof else None
+    traffic = traffic_pass / launches if traffic_pass else None           # HBM bytes per launch
     hbm = {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
-           "alg_bytes_per_pass": ALG_BYTES_PER_EVEN * evens, "traffic_bytes_per_pass": traffic}
+           "alg_bytes_per_pass": ALG_BYTES_PER_EVEN * evens, "alg_bytes_per_launch": ALG_BYTES_PER_EVEN * evens / launches,
+           "traffic_bytes_per_pass": traffic_pass, "traffic_bytes_per_launch": traffic}
     if prof and prof.get("sq_insts_valu_per_pass"):
         # the binding resource: VALU issue (counted wave-instructions over the live time)
         achieved_valu = prof["sq_insts_valu_per_pass"] / (kms * 1e-3)
