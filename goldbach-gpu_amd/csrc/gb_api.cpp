@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -513,6 +514,9 @@ int finish_segment(gb_ctx* c, uint32_t slot, uint32_t slow_cap, StatRec* out, do
         HCK(c, e);
         if (*seq != S.seq) return fail(c, GB_EDEVICE, "plain segment ended without its completion record");
     }
+    // the record and clocks were written before the sequence word: no read of them may move above
+    // the loop that saw it
+    std::atomic_thread_fence(std::memory_order_acquire);
     const double f = (double)(S.clk->t1 - S.clk->t0) / c->wall_khz;
     if (ms) *ms = f;
     if (c->cfg.flags & GB_CFG_KERNEL_TIMES) {
@@ -744,6 +748,8 @@ int gb_close(gb_ctx* c) {
                     c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0, c->d_tinfo};
     for (void* b : bufs)
         if (b) dfree(c, b);
+    for (auto& kv : c->held) (void)hipFree(kv.first);   // anything a failed call left behind
+    c->held.clear();
     if (c->h_out) dhostfree(c->h_out);
     if (c->h_wins) dhostfree(c->h_wins);
     if (c->h_lohi) dhostfree(c->h_lohi);

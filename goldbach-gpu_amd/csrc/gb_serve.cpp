@@ -265,6 +265,9 @@ void engine_main(gb_server* s, uint32_t eng) {
                     return;
                 }
             }
+            // the answers were written before the sequence word: none of their reads may move above
+            // the loop that saw it
+            std::atomic_thread_fence(std::memory_order_acquire);
             L.in_flight = false;
             retire(s, eng, L, results, L.launched_ns);
         }
@@ -541,7 +544,12 @@ int gb_serve_close(gb_server* s) {
 
 const char* gb_serve_last_error(const gb_server* s) {
     if (!s) return "";
-    return s->err.c_str();
+    // an engine may set the message meanwhile: the caller gets this thread's copy, valid until
+    // its next call
+    thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(const_cast<gb_server*>(s)->mu);
+    copy = s->err;
+    return copy.c_str();
 }
 
 }  // extern "C"

@@ -392,6 +392,7 @@ int gb_serve_poll(gb_server *srv, gb_result *out, uint32_t max, uint32_t *got, u
 int gb_serve_stats_get(gb_server *srv, gb_serve_stats *st);
 /* Stops the engine (queries not yet finished are dropped) and frees it. */
 int gb_serve_close(gb_server *srv);
+/* The server's last error message; the pointer stays valid until the calling thread's next call. */
 const char *gb_serve_last_error(const gb_server *srv);
 
 #ifdef __cplusplus
