@@ -144,6 +144,30 @@ def test_windows_match_oracle(ctx, orc):
     assert tot["evens"] == evens
 
 
+def test_many_ragged_windows_span_several_launches(ctx, orc):
+    # more windows than one launch takes pieces (64), of ragged lengths down to a single even n,
+    # below the wheel bound, across 2^44 - 2^24 and 2^44, and in the Baillie-PSW range; one window
+    # listed twice and two overlapping
+    rng = np.random.default_rng(7)
+    wins = []
+    for lo, hi, k, emax in ((4, 2**40, 60, 3000), (2**44 - 2**25, 2**44 + 2**22, 30, 3000),
+                            (10**15, 2**64 - 2 - 2 * 3000, 60, 2000)):
+        a = rng.integers(lo // 2, hi // 2, size=k, dtype=np.uint64) * np.uint64(2)
+        e = rng.integers(1, emax + 1, size=k)
+        wins += [(int(x), int(y)) for x, y in zip(a, e)]
+    wins += [wins[5], (wins[70][0] + 2 * 100, 500), (4, 1), (6, 1)]
+    per, tot = ctx.verify_windows(wins, 10**6, want_pmin=True)
+    for (a, e), got in zip(wins, per):
+        st, _, _ = orc.verify_segment(a, e, 10**6, want_bits=False, want_pmin=False)
+        for k in ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n"):
+            assert got[k] == st[k], (a, e, k)
+    assert tot["evens"] == sum(e for _, e in wins)
+    # without p_min statistics the fast marking path runs; its counts must not change
+    per2, _ = ctx.verify_windows(wins, 10**6)
+    for got, want in zip(per2, per):
+        assert (got["evens"], got["phase2"], got["failures"]) == (want["evens"], want["phase2"], want["failures"])
+
+
 def test_autotune_picks_a_valid_tuning(ctx, gb):
     want = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_PLAIN, seg_evens=1 << 27)
     t, rate = ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED)
