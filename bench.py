@@ -3,7 +3,7 @@
 
 One step = one complete verification pass over all 499,999,999,999 even n in
 [4, 1e12] with P_SMALL = 1e6, in the batched persistent form (one fused
-sieve+mark kernel, two workgroups per CU, over mod-6 wheel tiles of 3 * 2^18 even n
+sieve+mark kernel, two workgroups per CU, over mod-6 wheel tiles of 3 * 9 * 2^15 even n
 from a device queue, in launches of at most 2^17 tiles, each replayed as a
 hipGraph).  The plain per-segment form (2^27 evens per segment, as the reference's
 loop) is timed on the same range beside it.
@@ -144,7 +144,7 @@ def _kernel_stats(out_dir: str, name: str, key: str) -> dict:
     for r in _csv_rows(out_dir, name, "kernel_stats"):
         if key in r["Name"]:
             return {"calls": int(r["Calls"]), "total_ms": float(r["TotalDurationNs"]) * 1e-6,
-                    "avg_ms": float(r["AverageNs"]) * 1e-6}
+                    "avg_ms": float(r["AverageNs"]) * 1e-6, "name": r["Name"].split("(")[0].replace("void ", "")}
     raise SystemExit(f"{key} not in the kernel stats of {name}")
 
 
@@ -173,8 +173,8 @@ def summarize_profiles(out_dir: str) -> int:
     fetch, write = summed("f", wheel)["FETCH_SIZE"], summed("w", wheel)["WRITE_SIZE"]
     hbm = fetch * 1024 * fetch_scale + write * 1024 * write_scale
     prof = {
-        "kernel": "gbk::wheel_kernel<512, 262144u>",
-        "workload": "all even n in [4, 1e12], batched persistent form, mod-6 wheel tiles of 3 x 2^18 evens, two "
+        "kernel": trace["name"],
+        "workload": "all even n in [4, 1e12], batched persistent form, mod-6 wheel tiles of 3 x 9 x 2^15 evens, two "
                     "workgroups per CU, P_SMALL = 1e6: one pass = %d launches of at most 2^17 tiles" % trace["calls"],
         "build_hash": build_hash(),
         "launches_per_pass": trace["calls"],
@@ -776,7 +776,7 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                     "valu": None, "valu_note": "no counter profile of the current sources"}
-    roofline.update({"kernel": "wheel_kernel<512, 262144>", "launches_per_pass": launches, "kernel_ms_per_pass": kms,
+    roofline.update({"kernel": "wheel_kernel<512, 294912>", "launches_per_pass": launches, "kernel_ms_per_pass": kms,
                      "kernel_ms_per_launch": kms / launches, "build_hash": build_hash(),
                      "profile": os.path.relpath(TRAFFIC_FILE, ROOT)})
 
@@ -822,7 +822,7 @@ def main() -> int:
         "dtype": "u64",
         "data": "synthetic (the input is the integer range itself)",
         "config": {"workload": "all even n in [4, 1e12], batched persistent form", "limit": LIMIT,
-                   "p_small": P_SMALL, "tile_evens": 3 << 18, "workgroups_per_cu": 2,
+                   "p_small": P_SMALL, "tile_evens": 27 << 15, "workgroups_per_cu": 2,
                    "parallelism": "1 GPU",
                    "plain_seg_evens": PLAIN_SEG},
         "roofline": roofline,

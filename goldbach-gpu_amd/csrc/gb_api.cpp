@@ -31,7 +31,7 @@ constexpr uint64_t DEFAULT_P2 = 100000000ull;           // goldbach.cu:316
 constexpr uint64_t VRAM_MARGIN = 50ull << 20;            // goldbach.cu:96
 constexpr uint32_t MARK_WPB = 2048;                      // plain marking: verified words per workgroup
 constexpr uint32_t DEFAULT_TILE_LOG2 = 20;               // fused tile: 2^20 evens
-constexpr uint32_t WHEEL_TILE_LOG2 = 19;                 // wheel default: 3 * 2^18 evens, two workgroups per CU
+constexpr uint32_t WHEEL_TILE_LOG2 = 19;                 // wheel default: 3 * 9 * 2^15 evens, two workgroups per CU
 constexpr uint32_t SLOW_CAP = 1u << 20;
 constexpr uint32_t SEED_LIMIT = 4095;                    // trial-division seed (covers sqrt(2^22))
 constexpr uint32_t SP_TABLE_CAP = 300000;                // odd primes below 2^22: 295,946
@@ -120,13 +120,13 @@ void dfree(gb_ctx* c, void* p) {
 }
 
 // Medium-prime schedule for tiles of at most `tb` bits: prime k with
-// p*med_div < tb (med_div = 32 by default, 96 in the wheel) is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK
+// p*med_div < tb (med_div = 32 by default, 128 in the wheel) is cut into ceil(tb/(CHUNK_SPAN*p)) items of CHUNK
 // marks (an item skips the multiples of 3, so its CHUNK marks span CHUNK_SPAN*p bits); the
 // primes above get a thread each.
 //
 // Wheel form (wheel = true): tb is the bit count of each stream; items are
 // encoded (k << 13) | (stream << 12) | chunk and the streams hold no multiples
-// of 3.  Primes with p*wave_div < tb (wave_div = 2048 by default: at least 4 * 64 * WAVE_MARKS
+// of 3.  Primes with p*wave_div < tb (wave_div = 1024 by default: at least 2 * 64 * WAVE_MARKS
 // marks per stream) get wave items: ceil(tb/(64*WAVE_MARKS*p)) per stream, one wavefront each,
 // lane l marking l, l + 64, ... of the item's 64*WAVE_MARKS marks; the others
 // get ceil(tb/(CHUNK*p)) lane items of CHUNK consecutive marks per stream.
@@ -889,7 +889,7 @@ constexpr size_t GRAPH_CACHE = 8;
 
 struct Shape {
     bool wheel;
-    uint32_t tile_log2;   // fused: tiles of 2^tile_log2 evens; wheel: 3 * 2^(tile_log2 - 1) evens
+    uint32_t tile_log2;   // fused: tiles of 2^tile_log2 evens; wheel: 3 * wheel_class_len(tile_log2) evens
     uint32_t wg;          // persistent workgroups per CU
     uint32_t med_div = 0;    // work-item bounds of the medium-prime schedule (0 = defaults)
     uint32_t wave_div = 0;
@@ -913,7 +913,7 @@ Shape shape_of(bool wheel, uint32_t tile_log2, uint32_t wg, uint32_t med_div = 0
 int check_shape(gb_ctx* c, const Shape& sh) {
     if (sh.tile_log2 < 12 || sh.tile_log2 > 20) return fail(c, GB_EINVAL, "tile_log2 must be in [12, 20]");
     if (sh.wg > 16) return fail(c, GB_EINVAL, "wg_per_cu must be at most 16");
-    if (sh.wheel && sh.wg >= 2 && (1u << (sh.tile_log2 - 1)) > WH_CLASS_HALF)
+    if (sh.wheel && sh.wg >= 2 && wheel_class_len(sh.tile_log2) > WH_CLASS_HALF)
         return fail(c, GB_EINVAL, "two workgroups per CU take tiles of at most 2^19 evens");
     if (sh.med_div && (sh.med_div < 8 || sh.med_div > 256)) return fail(c, GB_EINVAL, "lane_item_div must be in [8, 256]");
     if (sh.wave_div && (sh.wave_div < 512 || sh.wave_div > 65536))
@@ -922,7 +922,7 @@ int check_shape(gb_ctx* c, const Shape& sh) {
 }
 
 uint64_t tile_span(const Shape& sh) {   // n covered by one tile
-    return sh.wheel ? (6ull << (sh.tile_log2 - 1)) : (2ull << sh.tile_log2);
+    return sh.wheel ? 6ull * wheel_class_len(sh.tile_log2) : (2ull << sh.tile_log2);
 }
 
 uint64_t piece_tiles(const Shape& sh, uint64_t start, uint64_t evens) {
@@ -964,12 +964,14 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
                double* ms, bool* overflow) {
     hipStream_t s = c->stream;
     const MarkParams mp = mark_params(c, p_small, flags);
-    const uint32_t cls_len = 1u << (sh.tile_log2 - 1), te = 1u << sh.tile_log2;
+    const uint32_t cls_len = wheel_class_len(sh.tile_log2), te = 1u << sh.tile_log2;
     ItemTable it;
     int rc = sh.wheel ? build_items(c, cls_len + WH_HALO + 64, &it, true, sh.med_div, sh.wave_div)
                       : build_items(c, te + HALO + WIN_PAD, &it, false, sh.med_div);
     if (rc) return rc;
     const size_t lds = sh.wheel ? wheel_lds_bytes(it.n_med, sh.wg) : fused_lds_bytes(te, it.n_med);
+    // (a schedule whose LDS lets fewer workgroups share a CU than asked runs correctly, with the
+    // workgroups that do not fit starting as others finish; gb_autotune skips such shapes)
     if (lds > LDS_PER_CU)
         return fail(c, GB_EINVAL, "the work items of this lane_item_div need more LDS than a CU holds at this tile size");
     const Tables T = make_tables(c, it);
@@ -1266,7 +1268,7 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
     if (form == GB_FORM_BATCHED) {
         for (uint32_t tl = 16; tl <= 20; ++tl) {
             ItemTable it;
-            rc = wheel ? build_items(c, (1u << (tl - 1)) + WH_HALO + 64, &it, true)
+            rc = wheel ? build_items(c, wheel_class_len(tl) + WH_HALO + 64, &it, true)
                        : build_items(c, (1u << tl) + HALO + WIN_PAD, &it);
             if (rc) return rc;
             for (uint32_t wg = 1; wg <= 2; ++wg) {
@@ -1330,8 +1332,9 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
         // second round: the medium primes' work-item bounds on the fastest tile shape
         const gb_tuning shape = pick;
         const uint32_t tl = shape.tile_log2;
-        const uint32_t tb = wheel ? (1u << (tl - 1)) + WH_HALO + 64 : (1u << tl) + HALO + WIN_PAD;
-        const std::vector<uint32_t> waves = wheel ? std::vector<uint32_t>{1024u, WAVE_ITEM_DIV} : std::vector<uint32_t>{0u};
+        const uint32_t tb = wheel ? wheel_class_len(tl) + WH_HALO + 64 : (1u << tl) + HALO + WIN_PAD;
+        const std::vector<uint32_t> waves = wheel ? std::vector<uint32_t>{WAVE_ITEM_DIV, 2 * WAVE_ITEM_DIV}
+                                                  : std::vector<uint32_t>{0u};
         std::vector<gb_tuning> items;
         const uint32_t md0 = wheel ? WHEEL_LANE_ITEM_DIV : LANE_ITEM_DIV;
         for (uint32_t md : {md0 / 2, md0, md0 * 2})

@@ -118,8 +118,13 @@ hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, co
 // Wheel (mod 6) form of the fused fast path: tiles of 3 * cls_len even n (cls_len per
 // class n mod 6), the window kept as two streams of q = 6k + 1 and q = 6k + 5.
 constexpr uint32_t WH_HALO = 1408;                   // k below n0 / 6 (covers p <= 8191: 1366)
-constexpr uint32_t WH_CLASS_MAX = 1u << 19;          // n per class per tile, at most (one workgroup per CU)
-constexpr uint32_t WH_CLASS_HALF = 1u << 18;         // ... with two half-size workgroups per CU
+// n per class of a wheel tile of size index t: 9 * 2^(t - 4), so a tile spans 27 * 2^(t - 3) n.
+// At t = 19 two workgroups' streams (2 x 74,096 B) and their dynamic LDS fill a CU's 160 KB;
+// a class length that is a multiple of 2^15 gives each of a 512-thread workgroup's threads the
+// same number of words to mark.
+__host__ __device__ constexpr uint32_t wheel_class_len(uint32_t t) { return 9u << (t - 4); }
+constexpr uint32_t WH_CLASS_MAX = wheel_class_len(20);    // n per class per tile, at most (one workgroup per CU)
+constexpr uint32_t WH_CLASS_HALF = wheel_class_len(19);   // ... with two half-size workgroups per CU
 // u32 per stream for tiles of at most cmax n per class
 __host__ __device__ constexpr uint32_t wh_stream_words(uint32_t cmax) { return (cmax + WH_HALO + 64) / 32 + 2; }
 size_t wheel_lds_bytes(uint32_t n_med, uint32_t wg_per_cu);

@@ -269,10 +269,10 @@ int main(int argc, char** argv) {
             return 1;
         }
         if (form == GB_FORM_BATCHED)
-            std::cout << "Autotuned      : tile 2^" << tun.tile_log2 << " evens, " << tun.wg_per_cu
+            std::cout << "Autotuned      : tile size index " << tun.tile_log2 << ", " << tun.wg_per_cu
                       << " workgroup(s) per CU, lane items below stream/"
-                      << (tun.lane_item_div ? tun.lane_item_div : 96u) << ", wave items below stream/"
-                      << (tun.wave_item_div ? tun.wave_item_div : 2048u);
+                      << (tun.lane_item_div ? tun.lane_item_div : 128u) << ", wave items below stream/"
+                      << (tun.wave_item_div ? tun.wave_item_div : 1024u);
         else
             std::cout << "Autotuned      : segments of " << tun.seg_evens << " evens";
         std::cout << " (" << std::scientific << std::setprecision(3) << rate << std::defaultfloat

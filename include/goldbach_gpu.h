@@ -156,18 +156,18 @@ int gb_kernel_times_get(gb_ctx *ctx, gb_kernel_times *out, int reset);
 
 typedef struct gb_tuning {
     uint32_t form;          /* GB_FORM_* */
-    uint32_t tile_log2;     /* batched: tile size 2^tile_log2 (0 with wg_per_cu 0: 19 with two
+    uint32_t tile_log2;     /* batched: tile size index t in 12..20 (0 with wg_per_cu 0: 19 with two
                                workgroups per CU for the part at or below 2^44 - 2^24, where tiles
-                               use the mod-6 wheel and hold 1.5 * 2^tile_log2 evens, and 20 with
-                               one above; a value given applies to both parts) */
+                               use the mod-6 wheel and hold 27 * 2^(t - 4) evens, and 20 with one
+                               above, where tiles hold 2^t evens; a value given applies to both parts) */
     uint64_t seg_evens;     /* plain: evens per segment (0 = 2^27) */
     uint32_t wg_per_cu;     /* batched: persistent workgroups per CU (0 = default; 2 needs tile_log2 <= 19) */
     uint32_t flags;         /* GB_WANT_PMIN | GB_TUNE_AUTO */
     uint32_t lane_item_div; /* batched: primes p with p * lane_item_div below a tile's stream bits are
                                cut into work items of 32 consecutive marks, larger ones get a thread
-                               each (0 = 96 in the wheel, 32 in odd-only tiles; 8..256) */
+                               each (0 = 128 in the wheel, 32 in odd-only tiles; 8..256) */
     uint32_t wave_item_div; /* batched wheel: primes p with p * wave_item_div below the stream bits
-                               are cut into wavefront items of 512 marks (0 = 2048; 512..65536) */
+                               are cut into wavefront items of 512 marks (0 = 1024; 512..65536) */
 } gb_tuning;
 
 #define GB_TUNE_AUTO 0x2u   /* gb_tuning.flags: take tile / segment size from gb_autotune (cached per context) */
@@ -216,7 +216,7 @@ typedef struct gb_chunk {
     uint64_t first_n, last_n;   /* even n covered */
     uint64_t tiles;             /* batched: tiles of the launch (0 for a plain segment) */
     uint32_t kernel;            /* GB_KERNEL_* */
-    uint32_t tile_log2;         /* batched: fused tiles of 2^t evens, wheel tiles of 3 * 2^(t-1) */
+    uint32_t tile_log2;         /* batched: fused tiles of 2^t evens, wheel tiles of 27 * 2^(t-4) */
     uint32_t wg_per_cu;
     uint32_t reserved;
 } gb_chunk;

@@ -14,6 +14,7 @@ pytestmark = pytest.mark.gpu
 
 WHEEL_N_MAX = (1 << 44) - (1 << 24)
 CHUNK_TILES = 1 << 17
+WHEEL_SPAN = 6 * (9 << 15)   # n spanned by a default wheel tile: 3 classes of 9 * 2^15 even n
 KEYS = ("evens", "phase2", "failures", "first_fail_n", "sum_pmin", "max_pmin", "argmax_n")
 
 
@@ -30,9 +31,9 @@ def test_plan_of_1e13_is_bounded_wheel_launches(ctx, gb):
     assert covers(plan, 4, 10**13)
     assert all(c["kernel"] == gb.GB_KERNEL_WHEEL and c["tiles"] <= CHUNK_TILES for c in plan)
     assert (plan[0]["tile_log2"], plan[0]["wg_per_cu"]) == (19, 2)
-    # whole chunks are whole numbers of 3 * 2^18-even tiles: every chunk after the first starts a tile
-    assert all(c["first_n"] % (6 << 18) == 0 for c in plan[1:])
-    assert len(plan) == -(-(10**13 // (6 << 18) + 1) // CHUNK_TILES)
+    # whole chunks are whole numbers of tiles: every chunk after the first starts a tile
+    assert all(c["first_n"] % WHEEL_SPAN == 0 for c in plan[1:])
+    assert len(plan) == -(-(10**13 // WHEEL_SPAN + 1) // CHUNK_TILES)
 
 
 def test_plan_splits_at_the_wheel_bound(ctx, gb):
@@ -60,8 +61,8 @@ def test_range_across_the_wheel_bound_matches_oracle(ctx, gb, orc):
 
 
 def test_1e13_within_a_budget_per_tile_records_would_exceed(gb):
-    # one 64-byte record per wheel tile over [4, 1e13] would take 407 MB; the context may hold 256 MiB
-    tiles = 10**13 // (6 << 18) + 1
+    # one 64-byte record per wheel tile over [4, 1e13] would take 362 MB; the context may hold 256 MiB
+    tiles = 10**13 // WHEEL_SPAN + 1
     assert tiles * 64 > 256 << 20
     with gb.Context(0, mem_budget=256 << 20) as c:
         b = c.verify_range(4, 10**13, 10**6)

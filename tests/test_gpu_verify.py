@@ -172,7 +172,7 @@ def test_autotune_picks_a_valid_tuning(ctx, gb):
     want = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_PLAIN, seg_evens=1 << 27)
     t, rate = ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED)
     assert 16 <= t["tile_log2"] <= 20 and t["wg_per_cu"] in (1, 2) and rate > 1e10
-    assert t["lane_item_div"] in (0, 48, 96, 192) and t["wave_item_div"] in (0, 1024, 2048)
+    assert t["lane_item_div"] in (0, 64, 128, 256) and t["wave_item_div"] in (0, 1024, 2048)
     assert ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED)[0] == t   # cached
     got = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED, autotune=True)
     assert (got["evens"], got["phase2"], got["failures"]) == (want["evens"], want["phase2"], want["failures"])

@@ -41,7 +41,7 @@ def test_first_tiles_and_tiny_ranges(ctx, gb, orc, start, limit):
 
 @pytest.mark.parametrize("tile_log2", [12, 13, 16, 19, 20])
 def test_tile_edges(ctx, gb, orc, tile_log2):
-    cls = 1 << (tile_log2 - 1)            # n per class per tile: a tile spans 6 * cls
+    cls = 9 << (tile_log2 - 4)            # n per class per tile: a tile spans 6 * cls
     for k in (1, 2, 3):
         check_range(ctx, gb, orc, 4, 6 * cls * k - 2, tile_log2=tile_log2)
         check_range(ctx, gb, orc, 4, 6 * cls * k, tile_log2=tile_log2)
@@ -77,7 +77,7 @@ def test_windows_around_the_wheel_edge(ctx, orc):
 @pytest.mark.parametrize("tile_log2", [12, 16, 19])
 def test_two_workgroups_per_cu(ctx, gb, orc, tile_log2):
     # half-size workgroups, two per CU (the default below 2^44 is tile_log2 19 this way)
-    cls = 1 << (tile_log2 - 1)
+    cls = 9 << (tile_log2 - 4)
     for start, limit in ((4, 6 * cls * 3), (1_000_002, 1_000_002 + 6 * cls * 2 + 10)):
         ref = orc.verify_range(start, limit, 1 << 22, 10**6)
         got = ctx.verify_range(start, limit, 10**6, form=gb.GB_FORM_BATCHED, tile_log2=tile_log2, wg_per_cu=2,
