@@ -467,9 +467,8 @@ SegPlan plan_segment(gb_ctx* c, uint64_t A, uint64_t cnt, WinDesc* desc) {
 
 // Queues one plain segment on buffers that are already large enough, staged through slot
 // `slot` (its descriptor already written by plan_segment); allocates nothing and does not wait.
-int enqueue_segment(gb_ctx* c, const SegBufs& B, const SegPlan& sp, uint32_t slot, uint64_t A, uint64_t cnt,
-                    const MarkParams& mp, bool want_verified, bool want_pmin) {
-    hipStream_t s = c->stream;
+int enqueue_segment(gb_ctx* c, hipStream_t s, const SegBufs& B, const SegPlan& sp, uint32_t slot, uint64_t A,
+                    uint64_t cnt, const MarkParams& mp, bool want_verified, bool want_pmin) {
     gb_ctx::SegSlot& S = c->seg_slot[slot];
     const ItemTable* it = nullptr;       // the plain tile schedule, built by gb_open
     for (const ItemTable& t : c->items)
@@ -501,15 +500,17 @@ int enqueue_segment(gb_ctx* c, const SegBufs& B, const SegPlan& sp, uint32_t slo
     return GB_OK;
 }
 
-// Waits for the segment queued in `slot`: its record to *out, its kernel time to *ms.
-int finish_segment(gb_ctx* c, uint32_t slot, uint32_t slow_cap, StatRec* out, double* ms) {
+// Waits for the segment queued in `slot` on stream s: its record to *out, its kernel time to *ms
+// and (clk, nullable) the device clock at its start and end.
+int finish_segment(gb_ctx* c, hipStream_t s, uint32_t slot, uint32_t slow_cap, StatRec* out, double* ms,
+                   uint64_t* clk = nullptr) {
     gb_ctx::SegSlot& S = c->seg_slot[slot];
     // spin on the pinned sequence word; now and then ask the stream, so a device error ends the
     // wait instead of leaving it spinning
     const volatile uint64_t* seq = &S.clk->seq;
     for (uint32_t spin = 1; *seq != S.seq; ++spin) {
         if (spin % 512) continue;
-        const hipError_t e = hipStreamQuery(c->stream);
+        const hipError_t e = hipStreamQuery(s);
         if (e == hipErrorNotReady) continue;
         HCK(c, e);
         if (*seq != S.seq) return fail(c, GB_EDEVICE, "plain segment ended without its completion record");
@@ -519,6 +520,10 @@ int finish_segment(gb_ctx* c, uint32_t slot, uint32_t slow_cap, StatRec* out, do
     std::atomic_thread_fence(std::memory_order_acquire);
     const double f = (double)(S.clk->t1 - S.clk->t0) / c->wall_khz;
     if (ms) *ms = f;
+    if (clk) {
+        clk[0] = S.clk->t0;
+        clk[1] = S.clk->t1;
+    }
     if (c->cfg.flags & GB_CFG_KERNEL_TIMES) {
         HCK(c, hipEventSynchronize(S.ev_rest));
         float fs = 0, fm = 0;
@@ -544,9 +549,61 @@ int finish_segment(gb_ctx* c, uint32_t slot, uint32_t slow_cap, StatRec* out, do
 int run_segment_on(gb_ctx* c, const SegBufs& B, uint64_t A, uint64_t cnt, const MarkParams& mp, bool want_verified,
                    bool want_pmin, double* ms, StatRec* out) {
     const SegPlan sp = plan_segment(c, A, cnt, c->seg_slot[0].h_win);
-    int rc = enqueue_segment(c, B, sp, 0, A, cnt, mp, want_verified, want_pmin);
+    int rc = enqueue_segment(c, c->stream, B, sp, 0, A, cnt, mp, want_verified, want_pmin);
     if (rc) return rc;
-    return finish_segment(c, 0, B.sl.cap, out, ms);
+    return finish_segment(c, c->stream, 0, B.sl.cap, out, ms);
+}
+
+void free_plain2(gb_ctx* c) {
+    gb_ctx::PlainSet& P = c->plain2;
+    void* bufs[] = {P.window, P.verified, P.rec, P.r0, P.wins, P.lohi, P.out, P.slow_n, P.slow_win, P.slow_res,
+                    P.slow_ctl};
+    for (void* b : bufs)
+        if (b) dfree(c, b);
+    const hipStream_t s = P.stream;
+    P = gb_ctx::PlainSet{};
+    P.stream = s;
+}
+
+// The second set of plain segment buffers for segments of up to cnt evens, allocated or grown
+// within the budget.  False (and no error) when it does not fit: the range then runs on one stream.
+bool ensure_plain2(gb_ctx* c, uint64_t cnt) {
+    gb_ctx::PlainSet& P = c->plain2;
+    const uint64_t wwords = (cnt + HALO + WIN_PAD + 63) / 64 + 1, vwords = (cnt + 63) / 64;
+    const uint64_t nrec = (vwords + MARK_WPB - 1) / MARK_WPB;
+    if (P.window && wwords <= P.window_cap && vwords <= P.verified_cap && nrec <= P.rec_cap) return true;
+    free_plain2(c);
+    const std::string err = c->err;
+    bool ok = (P.stream || hipStreamCreate(&P.stream) == hipSuccess) &&
+              dalloc(c, &P.window, wwords) == hipSuccess && dalloc(c, &P.verified, vwords) == hipSuccess &&
+              dalloc(c, &P.rec, nrec) == hipSuccess && dalloc(c, &P.r0, c->nsp) == hipSuccess &&
+              dalloc(c, &P.wins, 1) == hipSuccess && dalloc(c, &P.lohi, 2) == hipSuccess &&
+              dalloc(c, &P.out, 1) == hipSuccess && dalloc(c, &P.slow_n, SLOW_CAP) == hipSuccess &&
+              dalloc(c, &P.slow_win, SLOW_CAP) == hipSuccess && dalloc(c, &P.slow_res, SLOW_CAP) == hipSuccess &&
+              dalloc(c, &P.slow_ctl, 3) == hipSuccess;
+    if (!ok) {
+        free_plain2(c);
+        c->budget_hit = false;
+        c->err = err;
+        return false;
+    }
+    P.window_cap = wwords;
+    P.verified_cap = vwords;
+    P.rec_cap = nrec;
+    return true;
+}
+
+SegBufs plain2_bufs(const gb_ctx* c) {
+    const gb_ctx::PlainSet& P = c->plain2;
+    SlowList sl;
+    sl.n = P.slow_n;
+    sl.win = P.slow_win;
+    sl.count = reinterpret_cast<uint32_t*>(P.slow_ctl);
+    sl.overflow = reinterpret_cast<uint32_t*>(P.slow_ctl) + 2;
+    sl.cap = SLOW_CAP;
+    sl.pad = 0;
+    return SegBufs{P.window, P.window_cap, P.verified, P.verified_cap, nullptr, 0, P.rec, P.rec_cap, P.r0, c->nsp,
+                   P.wins, P.lohi, P.lohi + 1, P.out, sl, P.slow_res, P.slow_ctl};
 }
 
 // The context's buffers grown for plain segments of up to cnt evens (r0 to the whole table, so a
@@ -736,6 +793,7 @@ int gb_close(gb_ctx* c) {
     if (c->servers.load() > 0) return fail(c, GB_EAGAIN, "gb_close: close the context's servers first");
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->plain2.stream) (void)hipStreamSynchronize(c->plain2.stream);
     for (auto& g : c->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (auto& it : c->items) {
@@ -748,6 +806,8 @@ int gb_close(gb_ctx* c) {
                     c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0, c->d_tinfo};
     for (void* b : bufs)
         if (b) dfree(c, b);
+    free_plain2(c);
+    if (c->plain2.stream) (void)hipStreamDestroy(c->plain2.stream);
     for (auto& kv : c->held) (void)hipFree(kv.first);   // anything a failed call left behind
     c->held.clear();
     if (c->h_out) dhostfree(c->h_out);
@@ -1163,20 +1223,31 @@ int plan_batched(gb_ctx* c, uint64_t start, uint64_t limit, const gb_tuning& t, 
 // every segment, goldbach.cu:462-463).
 int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, const MarkParams& mp,
                      gb_seg_stats* tot, volatile uint64_t* progress) {
-    int rc = ensure_plain(c, std::min(seg, (limit - start) / 2 + 1), false);
+    const uint64_t first = std::min(seg, (limit - start) / 2 + 1);
+    int rc = ensure_plain(c, first, false);
     if (rc) return rc;
-    const SegBufs B = ctx_bufs(c);
+    // two segments in flight: on two streams and buffer sets when the second set fits (a range
+    // of one segment, or a context that times the kernels apart, keeps one stream)
+    SegBufs B[2] = {ctx_bufs(c), ctx_bufs(c)};
+    hipStream_t st[2] = {c->stream, c->stream};
+    if (first < (limit - start) / 2 + 1 && !(c->cfg.flags & GB_CFG_KERNEL_TIMES) && ensure_plain2(c, first)) {
+        B[1] = plain2_bufs(c);
+        st[1] = c->plain2.stream;
+    }
     uint64_t done = 0;
     uint64_t a = start, queued = 0, finished = 0;
     uint64_t cnt_of[2] = {0, 0};
     bool more = true;
+    // the device time of the range: the union of its segments' [start, end] clock intervals
+    uint64_t busy = 0, end_max = 0;
+    const double ms0 = tot->kernel_ms;
     auto queue_next = [&]() -> int {
         const uint64_t left = (limit - a) / 2 + 1;
         const uint64_t cnt = std::min(left, seg);
         const uint32_t slot = (uint32_t)(queued & 1);
         const SegPlan sp = plan_segment(c, a, cnt, c->seg_slot[slot].h_win);
         // the verified map is written, as the reference does
-        int r = enqueue_segment(c, B, sp, slot, a, cnt, mp, true, false);
+        int r = enqueue_segment(c, st[slot], B[slot], sp, slot, a, cnt, mp, true, false);
         if (r) return r;
         cnt_of[slot] = cnt;
         ++queued;
@@ -1191,18 +1262,25 @@ int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, const Mar
         const uint32_t slot = (uint32_t)(finished & 1);
         StatRec r;
         double ms = 0;
-        rc = finish_segment(c, slot, B.sl.cap, &r, &ms);
+        uint64_t clk[2] = {0, 0};
+        rc = finish_segment(c, st[slot], slot, B[slot].sl.cap, &r, &ms, clk);
         if (rc) break;
         ++finished;
         gb_seg_stats one{};
         to_stats(r, ms, &one);
+        const uint64_t from = std::max(clk[0], end_max);
+        if (clk[1] > from) busy += clk[1] - from;
+        end_max = std::max(end_max, clk[1]);
+        one.kernel_ms = 0;
         add_stats(tot, one);
+        tot->kernel_ms = ms0 + (double)busy / c->wall_khz;
         done += cnt_of[slot];
         if (progress) *progress = done;
         if (one.failures) break;        // the reference stops at the first failure (goldbach.cu:420,476-480)
     }
     // a segment queued past a failure or an error runs out; its results are not counted
-    HCK(c, hipStreamSynchronize(c->stream));
+    HCK(c, hipStreamSynchronize(st[0]));
+    HCK(c, hipStreamSynchronize(st[1]));
     return rc;
 }
 

@@ -105,6 +105,24 @@ struct gb_ctx {
     };
     SegSlot seg_slot[2];
     uint64_t seg_seq = 0;        // plain segments queued on this context
+    // A second set of plain segment buffers on a stream of its own, taken by a plain range of more
+    // than one segment: segment k + 1's sieve then runs beside segment k's marking.  Allocated on
+    // first use within the budget; without it (or with GB_CFG_KERNEL_TIMES) segments share one stream.
+    struct PlainSet {
+        hipStream_t stream = nullptr;
+        uint64_t* window = nullptr;
+        uint64_t* verified = nullptr;
+        gbk::StatRec* rec = nullptr;
+        gbk::PrimeRes* r0 = nullptr;
+        gbk::WinDesc* wins = nullptr;
+        uint64_t* lohi = nullptr;       // [rec_lo | rec_hi]
+        gbk::StatRec* out = nullptr;
+        uint64_t* slow_n = nullptr;
+        uint32_t* slow_win = nullptr;
+        uint64_t* slow_res = nullptr;
+        uint64_t* slow_ctl = nullptr;
+        uint64_t window_cap = 0, verified_cap = 0, rec_cap = 0;   // words, words, records
+    } plain2;
     double wall_khz = 0;         // the device wall clock's rate (wall_clock64 ticks per ms)
     void* h_slot_block = nullptr;
     gb_kernel_times ktimes{};
