@@ -23,7 +23,8 @@ The JSON line carries:
                 (the protocol of benchmarks/measure_variance.sh:11-28).
   valu_probe    issue rates of independent v_add_u32, independent v_mad_u64_u32
                 and a dependent v_add_u32 chain (lib/libgb_probe.so).
-  plain         the plain form over [4, 1e12] with its host overhead; beside it
+  plain         the plain form over [4, 1e12] (two segments in flight on two streams)
+                with its host overhead; beside it
                 [4, 1e11] in 2^27-even segments at P_SMALL = 1e6 and 4e9 (the cost
                 must not grow with P_SMALL), with the sieve and marking kernels
                 timed apart and priced against the HBM roofline, and [4, 1e9] in

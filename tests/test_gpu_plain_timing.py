@@ -1,10 +1,11 @@
 """The plain form's kernel time.  Plain segments end without an event between them: each
 segment's reduction writes the device wall clock at the segment's start and end and then a
 sequence number into pinned memory, and the host waits on that word.  So the kernel time a
-plain call reports comes from the device clock: it must be positive, no longer than the call,
+plain call reports comes from the device clock (the union of its segments' intervals, since two
+segments run side by side on two streams): it must be positive, no longer than the call,
 identical in its statistics to a context that also times the sieve and the marking apart
-(GB_CFG_KERNEL_TIMES), and equal there to the sum of that split.  (The reference reports
-only wall time, goldbach.cu:702-758.)"""
+(GB_CFG_KERNEL_TIMES, which keeps one stream), and equal there to the sum of that split.  (The
+reference reports only wall time, goldbach.cu:702-758.)"""
 import time
 
 import pytest
