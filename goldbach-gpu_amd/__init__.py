@@ -69,7 +69,7 @@ class GbSegStats(ctypes.Structure):
 
 class GbTuning(ctypes.Structure):
     _fields_ = [("form", u32), ("tile_log2", u32), ("seg_evens", u64), ("wg_per_cu", u32), ("flags", u32),
-                ("lane_item_div", u32), ("wave_item_div", u32)]
+                ("lane_item_div", u32), ("wave_item_div", u32), ("mr_proof_stride", u32)]
 
 
 class GbChunk(ctypes.Structure):
@@ -338,10 +338,10 @@ class Context:
     def verify_range(self, start: int, limit: int, p_small: int, form: int = GB_FORM_BATCHED,
                      tile_log2: int = 0, seg_evens: int = 0, wg_per_cu: int = 0, want_pmin: bool = False,
                      autotune: bool = False, progress: Optional[ctypes.c_uint64] = None,
-                     lane_item_div: int = 0, wave_item_div: int = 0) -> dict:
+                     lane_item_div: int = 0, wave_item_div: int = 0, mr_proof_stride: int = 0) -> dict:
         """progress (optional): a ctypes.c_uint64 the library updates with the evens finished."""
         flags = (GB_WANT_PMIN if want_pmin else 0) | (GB_TUNE_AUTO if autotune else 0)
-        tun = GbTuning(form, tile_log2, seg_evens, wg_per_cu, flags, lane_item_div, wave_item_div)
+        tun = GbTuning(form, tile_log2, seg_evens, wg_per_cu, flags, lane_item_div, wave_item_div, mr_proof_stride)
         st = GbSegStats()
         self._check(self._lib.gb_verify_range(self._h, start, limit, p_small, ctypes.byref(tun), ctypes.byref(st),
                                               ctypes.byref(progress) if progress is not None else None))

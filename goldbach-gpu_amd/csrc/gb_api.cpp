@@ -228,7 +228,7 @@ MarkParams mark_params(const gb_ctx* c, uint64_t p_small, uint32_t flags) {
     mp.K = count_le(c->h_sp, reach);
     mp.fast_final = mp.bound <= HALO_P ? 1u : 0u;
     mp.flags = flags & GB_WANT_PMIN;
-    mp.pad = 0;
+    mp.proof_stride = 1;
     return mp;
 }
 
@@ -944,6 +944,9 @@ int gb_verify_segment_noalloc(gb_ctx* c, uint64_t A, uint64_t cnt, uint64_t p_sm
 namespace {
 
 constexpr uint64_t CHUNK_TILES = 1ull << 17;   // tiles per launch at most: 8 MiB of records
+// Odd-only launches hold at most 2^15 tiles of 2^20 evens: their sparse proofs leave about 1e-5
+// of the n open, well within one launch's slow list of 2^20 (an overflowing launch runs again).
+constexpr uint64_t FUSED_CHUNK_TILES = 1ull << 15;
 constexpr uint32_t LAUNCH_PIECES = 64;          // pieces per launch at most
 constexpr size_t GRAPH_CACHE = 8;
 
@@ -953,6 +956,7 @@ struct Shape {
     uint32_t wg;          // persistent workgroups per CU
     uint32_t med_div = 0;    // work-item bounds of the medium-prime schedule (0 = defaults)
     uint32_t wave_div = 0;
+    uint32_t proof_stride = 0;   // odd-only MR tiles: survivors proven in one word in this many (0 = 4)
 };
 
 // Even n [start, start + 2(evens - 1)] that belong to output `owner`.
@@ -963,11 +967,12 @@ struct Piece {
 
 // Tile shape of a kind: the defaults when neither knob is given, else the knobs (one workgroup per
 // CU and 2^20-even tiles where only one of them is).
-Shape shape_of(bool wheel, uint32_t tile_log2, uint32_t wg, uint32_t med_div = 0, uint32_t wave_div = 0) {
+Shape shape_of(bool wheel, uint32_t tile_log2, uint32_t wg, uint32_t med_div = 0, uint32_t wave_div = 0,
+               uint32_t proof_stride = 0) {
     if (tile_log2 == 0 && wg == 0)
-        return wheel ? Shape{true, WHEEL_TILE_LOG2, 2, med_div, wave_div}
-                     : Shape{false, DEFAULT_TILE_LOG2, 1, med_div, wave_div};
-    return Shape{wheel, tile_log2 ? tile_log2 : DEFAULT_TILE_LOG2, wg ? wg : 1, med_div, wave_div};
+        return wheel ? Shape{true, WHEEL_TILE_LOG2, 2, med_div, wave_div, proof_stride}
+                     : Shape{false, DEFAULT_TILE_LOG2, 1, med_div, wave_div, proof_stride};
+    return Shape{wheel, tile_log2 ? tile_log2 : DEFAULT_TILE_LOG2, wg ? wg : 1, med_div, wave_div, proof_stride};
 }
 
 int check_shape(gb_ctx* c, const Shape& sh) {
@@ -978,8 +983,11 @@ int check_shape(gb_ctx* c, const Shape& sh) {
     if (sh.med_div && (sh.med_div < 8 || sh.med_div > 256)) return fail(c, GB_EINVAL, "lane_item_div must be in [8, 256]");
     if (sh.wave_div && (sh.wave_div < 512 || sh.wave_div > 65536))
         return fail(c, GB_EINVAL, "wave_item_div must be in [512, 65536]");
+    if (sh.proof_stride > 64) return fail(c, GB_EINVAL, "mr_proof_stride must be at most 64");
     return GB_OK;
 }
+
+uint64_t launch_tiles(const Shape& sh) { return sh.wheel ? CHUNK_TILES : FUSED_CHUNK_TILES; }
 
 uint64_t tile_span(const Shape& sh) {   // n covered by one tile
     return sh.wheel ? 6ull * wheel_class_len(sh.tile_log2) : (2ull << sh.tile_log2);
@@ -1023,7 +1031,15 @@ void drop_graphs(gb_ctx* c) {
 int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64_t p_small, uint32_t flags,
                double* ms, bool* overflow) {
     hipStream_t s = c->stream;
-    const MarkParams mp = mark_params(c, p_small, flags);
+    MarkParams mp = mark_params(c, p_small, flags);
+    // odd-only tiles with the fast marking: their strong tests may prove a subset of the survivors
+    // (the slow list then decides the n left open exactly)
+    const uint32_t stride = sh.proof_stride ? sh.proof_stride : MR_PROOF_STRIDE;
+    if (!sh.wheel && stride > 1 && mp.K == NMARK && mp.p_small >= HALO_P && !mp.fast_final &&
+        !(mp.flags & GB_WANT_PMIN)) {
+        mp.flags |= MP_SPARSE;
+        mp.proof_stride = stride;
+    }
     const uint32_t cls_len = wheel_class_len(sh.tile_log2), te = 1u << sh.tile_log2;
     ItemTable it;
     int rc = sh.wheel ? build_items(c, cls_len + WH_HALO + 64, &it, true, sh.med_div, sh.wave_div)
@@ -1077,7 +1093,7 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
     if (grid > tiles) grid = tiles;
     SlowList sl = slow_list(c);
     const std::vector<uint64_t> key = {sh.wheel ? 1ull : 0ull, sh.tile_log2, sh.wg, sh.med_div, sh.wave_div,
-                                       np, tiles, grid, p_small, flags,
+                                       mp.flags, mp.proof_stride, np, tiles, grid, p_small, flags,
                                        (uint64_t)(uintptr_t)c->d_rec, (uint64_t)(uintptr_t)c->d_tinfo,
                                        (uint64_t)(uintptr_t)c->d_wins,
                                        (uint64_t)(uintptr_t)c->d_out, (uint64_t)(uintptr_t)c->d_r0,
@@ -1147,7 +1163,7 @@ int run_pieces(gb_ctx* c, const Shape& sh, const std::vector<Piece>& pieces, uin
         uint64_t tiles = 0;
         while (j < pieces.size() && j - i < LAUNCH_PIECES) {
             const uint64_t nt = piece_tiles(sh, pieces[j].start, pieces[j].evens);
-            if (j > i && tiles + nt > CHUNK_TILES) break;
+            if (j > i && tiles + nt > launch_tiles(sh)) break;
             tiles += nt;
             ++j;
         }
@@ -1206,12 +1222,12 @@ int plan_batched(gb_ctx* c, uint64_t start, uint64_t limit, const gb_tuning& t, 
         const bool wheel = kinds[k].first;
         if (auto_pick && k == dom)
             P.sh = shape_of(wheel, auto_pick->tile_log2, auto_pick->wg_per_cu, auto_pick->lane_item_div,
-                            auto_pick->wave_item_div);
-        else if (auto_pick) P.sh = shape_of(wheel, 0, 0);
-        else P.sh = shape_of(wheel, t.tile_log2, t.wg_per_cu, t.lane_item_div, t.wave_item_div);
+                            auto_pick->wave_item_div, t.mr_proof_stride);
+        else if (auto_pick) P.sh = shape_of(wheel, 0, 0, 0, 0, t.mr_proof_stride);
+        else P.sh = shape_of(wheel, t.tile_log2, t.wg_per_cu, t.lane_item_div, t.wave_item_div, t.mr_proof_stride);
         int rc = check_shape(c, P.sh);
         if (rc) return rc;
-        cut_pieces(P.sh, kinds[k].second.first, kinds[k].second.second, 0, CHUNK_TILES, P.pieces);
+        cut_pieces(P.sh, kinds[k].second.first, kinds[k].second.second, 0, launch_tiles(P.sh), P.pieces);
         parts.push_back(std::move(P));
     }
     return GB_OK;
@@ -1377,10 +1393,10 @@ int gb_autotune(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t p_small, uin
             for (int rep = 0; rep < 3; ++rep) {
                 const auto t0 = std::chrono::steady_clock::now();
                 if (form == GB_FORM_BATCHED) {
-                    const Shape sh{wheel, t.tile_log2, t.wg_per_cu, t.lane_item_div, t.wave_item_div};
+                    const Shape sh{wheel, t.tile_log2, t.wg_per_cu, t.lane_item_div, t.wave_item_div, t.mr_proof_stride};
                     std::vector<Piece> pcs;
                     for (const WinDesc& w : sample)
-                        cut_pieces(sh, w.start, w.start + 2 * (w.evens - 1), 0, CHUNK_TILES, pcs);
+                        cut_pieces(sh, w.start, w.start + 2 * (w.evens - 1), 0, launch_tiles(sh), pcs);
                     gb_seg_stats sink{};
                     double ms = 0;
                     bool failed = false;
@@ -1551,7 +1567,8 @@ int gb_verify_windows(gb_ctx* c, const gb_window* windows, uint32_t count, uint6
         const uint64_t a = windows[i].start;
         split_kinds(a, a + 2 * (windows[i].evens - 1), kinds);
         for (auto& k : kinds)
-            cut_pieces(k.first ? shw : shf, k.second.first, k.second.second, i, CHUNK_TILES, k.first ? pw : pf);
+            cut_pieces(k.first ? shw : shf, k.second.first, k.second.second, i, launch_tiles(k.first ? shw : shf),
+                       k.first ? pw : pf);
     }
     std::vector<gb_seg_stats> per(count);
     double ms = 0;

@@ -20,6 +20,7 @@ constexpr uint32_t PLAIN_TILE_BITS = 1u << 18;   // plain sieve tile (bits per w
 constexpr uint32_t LANE_ITEM_DIV = 32;            // default: lane items for p * 32 < tile bits (odd-only tiles)
 constexpr uint32_t WHEEL_LANE_ITEM_DIV = 128;     // ... and p * 128 < stream bits in the wheel
 constexpr uint32_t WAVE_ITEM_DIV = 1024;          // default: wave items for p * 1024 < stream bits
+constexpr uint32_t MR_PROOF_STRIDE = 3;           // default: odd-only MR tiles prove one window word in 3
 
 struct ItemTable {
     uint32_t tb = 0;        // tile bits the schedule was cut for

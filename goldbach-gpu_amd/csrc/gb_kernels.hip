@@ -122,11 +122,19 @@ __device__ uint32_t block_scan(uint32_t c, uint32_t tid, uint32_t* __restrict__ 
 // a failed test clears the bit.  No composite below 2^64 passes both, so the answer is
 // the one of the Miller-Rabin bases 2..37; nearly every base-2 survivor is prime, and
 // one Lucas test (a product and a square per bit) costs about two base-2 tests.
+// With stride > 1 only the survivors in every stride-th word are tested; the other words are
+// cleared, so a q left unproven reads as composite and the marking resolves an n only through a
+// proven prime (each n still meets about 65 / stride proven q among its 1,027 window primes).
 template <int BS>
 __device__ void mr_tile(uint32_t* __restrict__ win32, uint32_t nbits, uint64_t qb, uint32_t tid,
-                        uint32_t* __restrict__ queue, uint32_t qcap, uint32_t* __restrict__ wsum) {
+                        uint32_t* __restrict__ queue, uint32_t qcap, uint32_t* __restrict__ wsum, uint32_t stride) {
     const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
+    if (stride > 1) {
+        for (uint32_t k = tid; k < nw64; k += BS)
+            if (k % stride) reinterpret_cast<uint64_t*>(win32)[k] = 0;
+        __syncthreads();
+    }
     for (uint32_t pass = 0; pass < 2; ++pass) {
         for (uint32_t wb = 0; wb < nw64; wb += BS) {
             const uint32_t k = wb + tid;
@@ -184,7 +192,8 @@ __device__ __forceinline__ T ld32(const T* __restrict__ a, uint32_t i) {
 template <int BS, bool MR = true>
 __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ scratch, uint32_t scratch_cap,
                            uint32_t* __restrict__ wsum, uint64_t qv0, uint32_t lead, uint64_t g0, uint32_t nbits,
-                           const Tables& T, uint32_t tid, const PrimeRes* __restrict__ pr, uint64_t rbase) {
+                           const Tables& T, uint32_t tid, const PrimeRes* __restrict__ pr, uint64_t rbase,
+                           uint32_t proof_stride = 1) {
     uint32_t* med_i0 = scratch;   // medium-prime first marks; the Miller-Rabin queue afterwards
     uint64_t* win64 = reinterpret_cast<uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
@@ -324,7 +333,8 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     // 4) above 2^44 the table cannot reach sqrt(q): survivors of the pre-sieve
     //    are decided by Baillie-PSW (exact below 2^64).
     if constexpr (MR) {
-        if (mr && any_valid) mr_tile<BS>(win32, nbits, qv0 + 2 * g0 - 2ull * lead, tid, scratch, scratch_cap, wsum);
+        if (mr && any_valid)
+            mr_tile<BS>(win32, nbits, qv0 + 2 * g0 - 2ull * lead, tid, scratch, scratch_cap, wsum, proof_stride);
     }
 }
 
@@ -772,7 +782,8 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
         const uint64_t rem = wd.evens - (uint64_t)tile_evens * lt;
         const uint32_t cnt = (uint32_t)(rem < tile_evens ? rem : tile_evens);
         const WinGeom g = window_geom(n0, (uint64_t)cnt + HALO + WIN_PAD);
-        sieve_tile<BS>(win32, scratch, FUSED_MRQ, wsum, g.qv0, g.lead, 0, g.nbits, T, tid, pr + wd.r0off, wd.rbase);
+        sieve_tile<BS>(win32, scratch, FUSED_MRQ, wsum, g.qv0, g.lead, 0, g.nbits, T, tid, pr + wd.r0off, wd.rbase,
+                       (mp.flags & MP_SPARSE) ? mp.proof_stride : 1u);
         LaneStats st = lane_stats_zero();
         mark_words<BS>(reinterpret_cast<const uint64_t*>(win32), n0, cnt, mp, T.sp, nullptr, nullptr, sl, lo, st,
                        tid);
@@ -1243,10 +1254,36 @@ __global__ void residue_wheel_kernel(Tables T, const WinDesc* __restrict__ wins,
 // ---------------------------------------------------------------------------
 // slow path: entries whose p_min exceeds the window's reach (p > sp[K-1])
 // ---------------------------------------------------------------------------
+__device__ int partition_chunk(const Tables& T, uint64_t n, uint64_t half, uint64_t bound, uint64_t base,
+                               uint32_t lane, uint64_t& res);
+
 __global__ void slow_kernel(Tables T, MarkParams mp, SlowList sl, uint64_t* __restrict__ res_p,
                             const WinDesc* __restrict__ wins, uint64_t* __restrict__ verified,
                             uint32_t* __restrict__ pmin_out, uint64_t pmin_n0) {
     const uint32_t cnt = *sl.count < sl.cap ? *sl.count : sl.cap;
+    if (mp.flags & MP_SPARSE) {
+        // sparse proofs: an n here may have a partner among the window primes whose q was left
+        // unproven, so a wave per n scans the candidates from p = 3, 64 at a time, as a single
+        // query does (base-2 filter, the other bases on the lowest survivor)
+        const uint32_t lane = threadIdx.x & 63;
+        const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+        for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < cnt; i += nwaves) {
+            const uint64_t n = sl.n[i];
+            uint64_t found = 0;
+            for (uint64_t base = 0;; base += 64)
+                if (partition_chunk(T, n, n / 2, mp.bound, base, lane, found) != 0) break;
+            if (lane == 0) {
+                res_p[i] = found;
+                const WinDesc wd = wins[sl.win[i]];
+                if (found && found <= mp.p_small && verified) {
+                    const uint64_t bit = wd.word0 * 64 + (n - wd.start) / 2;
+                    atomicOr((unsigned long long*)&verified[bit >> 6], 1ull << (bit & 63));
+                }
+                if (pmin_out) pmin_out[(n - pmin_n0) / 2] = (uint32_t)found;
+            }
+        }
+        return;
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint64_t n = sl.n[i];
         const uint64_t half = n / 2;
@@ -1832,7 +1869,9 @@ hipError_t launch_wheel(hipStream_t s, const Tables& T, const MarkParams& mp, co
 
 hipError_t launch_slow(hipStream_t s, const Tables& T, const MarkParams& mp, SlowList sl, uint64_t* res_p,
                        const WinDesc* wins, uint64_t* verified, uint32_t* pmin_out, uint64_t pmin_n0) {
-    hipLaunchKernelGGL(slow_kernel, dim3(256), dim3(256), 0, s, T, mp, sl, res_p, wins, verified, pmin_out, pmin_n0);
+    // a wave per n with sparse proofs: more waves, so the strong tests' latency is hidden
+    const uint32_t blocks = (mp.flags & MP_SPARSE) ? 2048u : 256u;
+    hipLaunchKernelGGL(slow_kernel, dim3(blocks), dim3(256), 0, s, T, mp, sl, res_p, wins, verified, pmin_out, pmin_n0);
     return hipGetLastError();
 }
 

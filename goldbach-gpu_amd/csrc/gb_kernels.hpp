@@ -29,9 +29,13 @@ struct MarkParams {
     uint64_t bound;       // max(p_small, phase2_limit)
     uint32_t K;           // odd marking primes sp[0..K), all <= min(bound, HALO_P)
     uint32_t fast_final;  // 1 when sp[0..K) reaches bound: unresolved n are failures
-    uint32_t flags;       // bit 0: p_min statistics
-    uint32_t pad;
+    uint32_t flags;       // bit 0: p_min statistics; MP_SPARSE: see proof_stride
+    uint32_t proof_stride;   // with MP_SPARSE (odd-only tiles above 2^44): only the survivors in every
+                             // proof_stride-th 64-bit word of a tile's window are proven prime; the n
+                             // no proven q resolves go to the slow list, which scans them from p = 3
+                             // with exact tests
 };
+constexpr uint32_t MP_SPARSE = 2u;
 
 struct SlowList {
     uint64_t* n;          // unresolved even n (p_min above the window's reach)

@@ -168,6 +168,11 @@ typedef struct gb_tuning {
                                each (0 = 128 in the wheel, 32 in odd-only tiles; 8..256) */
     uint32_t wave_item_div; /* batched wheel: primes p with p * wave_item_div below the stream bits
                                are cut into wavefront items of 512 marks (0 = 1024; 512..65536) */
+    uint32_t mr_proof_stride; /* batched, odd-only tiles above 2^44 without GB_WANT_PMIN: the strong
+                               tests prove the survivors of one 64-bit window word in every s
+                               (0 = 3; 1 = every survivor; at most 64); an n that no
+                               proven q resolves is decided by the exact slow path, so the counts
+                               do not depend on s */
 } gb_tuning;
 
 #define GB_TUNE_AUTO 0x2u   /* gb_tuning.flags: take tile / segment size from gb_autotune (cached per context) */

@@ -168,6 +168,24 @@ def test_many_ragged_windows_span_several_launches(ctx, orc):
         assert (got["evens"], got["phase2"], got["failures"]) == (want["evens"], want["phase2"], want["failures"])
 
 
+@pytest.mark.parametrize("stride", [1, 2, 3, 64])
+def test_mr_proof_stride_leaves_the_counts_alone(ctx, gb, orc, stride):
+    # odd-only tiles above 2^44 prove the survivors of one window word in `stride` (4 by default);
+    # an n that no proven q resolves goes to the slow path, which decides it exactly from p = 3
+    # (at stride 64 about a third of the n go there).  P_SMALL = 8193 makes the record n
+    # 3325581707333960528 (p_min 9781) a Phase-2 case.
+    rec = 3325581707333960528
+    for a, e, ps in ((10**15, 30_000, 10**6), (2**63 + 2, 20_000, 10**6), (2**64 - 2 - 2 * 19_999, 20_000, 10**6),
+                     (rec - 2 * 5000, 10_000, 8193)):
+        st, _, _ = orc.verify_segment(a, e, ps, want_bits=False, want_pmin=False)
+        got = ctx.verify_range(a, a + 2 * (e - 1), ps, mr_proof_stride=stride)
+        assert (got["evens"], got["phase2"], got["failures"]) == (st["evens"], st["phase2"], st["failures"]), (a, stride)
+        if a == rec - 2 * 5000:
+            assert st["phase2"] >= 1
+    with pytest.raises(gb.GoldbachError):
+        ctx.verify_range(10**15, 10**15 + 1000, 10**6, mr_proof_stride=65)
+
+
 def test_autotune_picks_a_valid_tuning(ctx, gb):
     want = ctx.verify_range(4, 10**11, 10**6, form=gb.GB_FORM_PLAIN, seg_evens=1 << 27)
     t, rate = ctx.autotune(4, 10**11, 10**6, form=gb.GB_FORM_BATCHED)
