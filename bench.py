@@ -35,8 +35,9 @@ The JSON line carries:
   cpu_baseline  the CPU restatement (oracle/, OpenMP over segments, the
                 reference algorithm: per-n ascending prime scan) timed on this
                 host over 64 sampled 2^24-even segments spread over [4, 1e12].
-  high_windows  64 windows above 1e15 (q by pre-sieve + Baillie-PSW) in one
-                persistent launch, evens/s, with its counted VALU issue against
+  high_windows  64 windows above 1e15 (q by pre-sieve + Baillie-PSW, the survivors of one
+                window word in three proven, the n left open decided exactly by the
+                slow path) in persistent launches, evens/s, with its counted VALU issue against
                 the probe's ceiling; its CPU baseline is the oracle's
                 Miller-Rabin restatement on the first 2^19 evens of each window.
   partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
@@ -201,23 +202,27 @@ def summarize_profiles(out_dir: str) -> int:
     runs = [(ctr, ns) for kname, ctr, ns in _dispatches(out_dir, "hs") if fused in kname]
     hs_all = summed("hs", fused)
     htrace = _kernel_stats(out_dir, "ht", fused)
-    main_ctr, main_ns = max(runs, key=lambda x: x[0]["SQ_INSTS_VALU"])   # the measured launch (the other: warm-up)
+    # dispatch order: the warm-up call (the windows cut to 2^20 evens: one launch), then the measured
+    # call's launches
+    main_ctr = {c: sum(ctr[c] for ctr, _ in runs[1:]) for c in runs[0][0]}
+    main_ns = sum(ns for _, ns in runs[1:])
     hprof = {
         "kernel": "gbk::fused_kernel<1024>",
         "workload": "64 windows of uniform 1e6..1e9 evens at even starts log-uniform in [1e15, 1.8e19] (numpy seed "
                     "0): pre-sieve with every odd prime below 2^22, base-2 strong test, strong Lucas test",
         "build_hash": build_hash(),
-        "sq_insts_valu_per_launch": main_ctr["SQ_INSTS_VALU"],
-        "sq_insts_valu_int64_per_launch": main_ctr["SQ_INSTS_VALU_INT64"],
-        "sq_insts_valu_int32_per_launch": main_ctr["SQ_INSTS_VALU_INT32"],
+        "measured_launches": len(runs) - 1,
+        "sq_insts_valu_per_call": main_ctr["SQ_INSTS_VALU"],
+        "sq_insts_valu_int64_per_call": main_ctr["SQ_INSTS_VALU_INT64"],
+        "sq_insts_valu_int32_per_call": main_ctr["SQ_INSTS_VALU_INT32"],
         "kernel_ms_under_profiler": round(main_ns * 1e-6, 3),
         "trace_launches": htrace["calls"], "trace_ms_all_launches": round(htrace["total_ms"], 3),
         "sq_insts_valu_int64_launches": hs_all["SQ_INSTS_VALU_INT64"],
         "sq_insts_valu_int32_launches": hs_all["SQ_INSTS_VALU_INT32"],
         "sq_insts_salu_launches": hs_all["SQ_INSTS_SALU"], "sq_insts_lds_launches": hs_all["SQ_INSTS_LDS"],
         "commands": [profile_command(n, "<dir>") for n in ("ht", "hs")],
-        "note": "the *_per_launch figures are the measured launch (the larger fused_kernel dispatch; the other is "
-                "the warm-up on the same windows cut to 2^20 evens); the *_launches figures sum both",
+        "note": "the *_per_call figures sum the measured call's fused_kernel dispatches (the first dispatch is the "
+                "warm-up on the same windows cut to 2^20 evens); the *_launches figures sum all of them",
     }
     with open(WINDOWS_PROFILE, "w") as f:
         json.dump(hprof, f, indent=1)
@@ -556,26 +561,32 @@ def presieve_density() -> float:
     return float(np.prod(1.0 - 1.0 / ps))
 
 
-def montgomery_products(wins) -> float:
+MR_PROOF_STRIDE = 3   # the tiles prove the survivors of one window word in three (gb_tuning default)
+
+
+def montgomery_products(wins, stride: int = MR_PROOF_STRIDE) -> float:
     """Model of the 64-bit Montgomery products the windows need (one product = one mulmod).
 
-    A window of e evens near q needs the primality of about e odd q; e * S of them survive the
-    pre-sieve (S = presieve_density()) and take the base-2 strong test, b - 2 products for q of
-    b bits (b - 3 squarings of the ladder over d, n - 1 = d 2^r with E[r] = 2, and E[r - 1] = 1
-    more); the e * 2/ln q primes among them also take the strong Lucas test, 2b - 7 products
-    (two per bit of m = (d - 1)/2, n + 1 = d 2^s, one for the top bit, one for W_d, E[s - 1] = 1).
+    A window of e evens near q holds about e odd q; e * S of them survive the pre-sieve
+    (S = presieve_density()), and the tiles test those in one window word in `stride`: the
+    base-2 strong test, b - 2 products for q of b bits (b - 3 squarings of the ladder over d,
+    n - 1 = d 2^r with E[r] = 2, and E[r - 1] = 1 more); the primes among them (2/ln q of the odd
+    q) also take the strong Lucas test, 2b - 7 products (two per bit of m = (d - 1)/2,
+    n + 1 = d 2^s, one for the top bit, one for W_d, E[s - 1] = 1).  The few n no proven q
+    resolves (about 1e-5 of them) take the slow path's tests, which the model leaves out.
     """
     dens = presieve_density()
     total = 0.0
     for a, e in wins:
         q = a + e
         b = q.bit_length()
-        total += e * dens * (b - 2) + e * 2.0 / math.log(q) * (2 * b - 7)
+        total += (e * dens * (b - 2) + e * 2.0 / math.log(q) * (2 * b - 7)) / stride
     return total
 
 
 def windows_leg(ctx, probe) -> dict:
-    """High windows: q decided by pre-sieve + Baillie-PSW, all windows in one persistent launch.
+    """High windows: q decided by pre-sieve + Baillie-PSW, all windows in persistent launches of
+    at most 2^15 tiles.
     Its roofline is VALU issue: the VALU wave-instructions the launch issues (rocprofv3 count
     in the committed profile, when that profile was taken with the current sources) over the
     live launch time, against the v_add_u32 issue rate the probe measures."""
@@ -592,19 +603,19 @@ def windows_leg(ctx, probe) -> dict:
     ksec = tot["kernel_ms"] * 1e-3
     prof = load_profile(WINDOWS_PROFILE)
     peak = probe["v_add_u32_independent"]["wave_insts_per_s"] if probe else None
-    achieved = prof["sq_insts_valu_per_launch"] / ksec if prof else None
+    achieved = prof["sq_insts_valu_per_call"] / ksec if prof else None
     # 64-bit integer instructions (v_mad_u64_u32, 64-bit adds) issue at half the v_add_u32 rate:
     # counted as two issue slots each, the launch's slots over the live time against the same peak
-    int64 = prof.get("sq_insts_valu_int64_per_launch") if prof else None
-    slots = (prof["sq_insts_valu_per_launch"] + int64) / ksec if int64 else None
+    int64 = prof.get("sq_insts_valu_int64_per_call") if prof else None
+    slots = (prof["sq_insts_valu_per_call"] + int64) / ksec if int64 else None
     return {"workload": "64 windows, uniform 1e6..1e9 evens, starts log-uniform in [1e15, 1.8e19] (seed 0)",
             "evens": evens, "value": evens / dt, "unit": "evens/s", "seconds": dt,
             "kernel_ms": tot["kernel_ms"], "phase2": tot["phase2"],
             "roofline": {"bound": "valu", "unit": "wave-instructions/s", "achieved": achieved, "peak": peak,
                          "frac": achieved / peak if (achieved and peak) else None,
                          "kernel": "fused_kernel<1024>",
-                         "counted": "SQ_INSTS_VALU of the launch, profiles/high_windows_mr.json (null when that "
-                                    "profile was taken with other sources)",
+                         "counted": "SQ_INSTS_VALU of the call's launches, profiles/high_windows_mr.json (null when "
+                                    "that profile was taken with other sources)",
                          "peak_note": "v_add_u32 issue rate of gb_probe_valu, measured in this run",
                          "issue_slots_per_s": slots,
                          "issue_slots_frac": slots / peak if (slots and peak) else None,
