@@ -192,13 +192,14 @@ int gb_autotune(gb_ctx *ctx, uint64_t start, uint64_t limit, uint64_t p_small, u
 /* All even n in [start, limit] (start even >= 4 after the reference's
  * adjustments, goldbach.cu:594,604-609).  Replaces main's worker spawn and
  * the segment queue (goldbach.cu:689, 761-773, 378-500).
- *   plain form:   one segment of seg_evens after another, one host sync each.
+ *   plain form:   segments of seg_evens, two in flight (on two streams when
+ *                 a second buffer set fits the budget), one host wait each.
  *   batched form: the range is split at 2^44 - 2^24 (below, tiles of the
  *                 mod-6 wheel kernel; above, the odd-only fused kernel with
  *                 exact 64-bit tests) and each part is cut into launches of
- *                 at most 2^17 tiles, so the device memory a call holds does
- *                 not grow with the range (the reference's fixed per-segment
- *                 buffers, goldbach.cu:378-417).
+ *                 at most 2^17 wheel or 2^15 odd-only tiles, so the device
+ *                 memory a call holds does not grow with the range (the
+ *                 reference's fixed per-segment buffers, goldbach.cu:378-417).
  * Both forms stop after the first segment or launch holding a failure, as
  * the reference stops its workers (goldbach.cu:420,476-480); stats then
  * cover the evens examined up to there.  progress_evens (nullable) is
