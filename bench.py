@@ -36,8 +36,9 @@ The JSON line carries:
                 reference algorithm: per-n ascending prime scan) timed on this
                 host over 64 sampled 2^24-even segments spread over [4, 1e12].
   high_windows  64 windows above 1e15 (q by pre-sieve + Baillie-PSW, the survivors of one
-                window word in three proven, the n left open decided exactly by the
-                slow path) in persistent launches, evens/s, with its counted VALU issue against
+                window word in five proven, the n left open decided by their other
+                survivors, then exactly by the slow path) in persistent launches, evens/s,
+                with its counted VALU issue against
                 the probe's ceiling; its CPU baseline is the oracle's
                 Miller-Rabin restatement on the first 2^19 evens of each window.
   partition_counts  c(n) for every even n in [4, 3e7] (gb_count_partitions) in
@@ -561,7 +562,7 @@ def presieve_density() -> float:
     return float(np.prod(1.0 - 1.0 / ps))
 
 
-MR_PROOF_STRIDE = 3   # the tiles prove the survivors of one window word in three (gb_tuning default)
+MR_PROOF_STRIDE = 5   # the tiles prove the survivors of one window word in five (gb_tuning default)
 
 
 def montgomery_products(wins, stride: int = MR_PROOF_STRIDE) -> float:
@@ -572,8 +573,9 @@ def montgomery_products(wins, stride: int = MR_PROOF_STRIDE) -> float:
     base-2 strong test, b - 2 products for q of b bits (b - 3 squarings of the ladder over d,
     n - 1 = d 2^r with E[r] = 2, and E[r - 1] = 1 more); the primes among them (2/ln q of the odd
     q) also take the strong Lucas test, 2b - 7 products (two per bit of m = (d - 1)/2,
-    n + 1 = d 2^s, one for the top bit, one for W_d, E[s - 1] = 1).  The few n no proven q
-    resolves (about 1e-5 of them) take the slow path's tests, which the model leaves out.
+    n + 1 = d 2^s, one for the top bit, one for W_d, E[s - 1] = 1).  The n no proven q
+    resolves (about 3e-4 of them at stride 5) take a few more tests each, which the model
+    leaves out (under 1 % of the products).
     """
     dens = presieve_density()
     total = 0.0

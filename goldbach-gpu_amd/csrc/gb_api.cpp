@@ -257,6 +257,16 @@ void add_stats(gb_seg_stats* a, const gb_seg_stats& b) {
     a->kernel_ms += b.kernel_ms;
 }
 
+int ensure_wcopy(gb_ctx* c, uint64_t words) {
+    if (words <= c->wcopy_cap) return GB_OK;
+    dfree(c, c->d_wcopy);
+    c->d_wcopy = nullptr;
+    c->wcopy_cap = 0;
+    HCK(c, dalloc(c, &c->d_wcopy, words));
+    c->wcopy_cap = words;
+    return GB_OK;
+}
+
 int ensure_r0(gb_ctx* c, uint64_t n) {
     if (n <= c->r0_cap) return GB_OK;
     dfree(c, c->d_r0);
@@ -803,7 +813,7 @@ int gb_close(gb_ctx* c) {
     }
     void* bufs[] = {c->d_sp, c->d_inv, c->d_c32, c->d_d5, c->d_p2bits, c->d_window, c->d_verified, c->d_pmin, c->d_rec,
                     c->d_out, c->d_slow_n, c->d_slow_win, c->d_slow_ctl, c->d_slow_res, c->d_wins, c->d_rec_lo,
-                    c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0, c->d_tinfo};
+                    c->d_rec_hi, c->d_counter, c->d_qn, c->d_qp, c->d_r0, c->d_tinfo, c->d_wcopy};
     for (void* b : bufs)
         if (b) dfree(c, b);
     free_plain2(c);
@@ -1080,24 +1090,30 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
         }
         nres += wd.np;
     }
-    const void* old_bufs[] = {c->d_rec, c->d_r0, c->d_tinfo};
+    uint64_t grid = (uint64_t)c->cus * sh.wg;
+    if (grid > tiles) grid = tiles;
+    const void* old_bufs[] = {c->d_rec, c->d_r0, c->d_tinfo, c->d_wcopy};
     rc = ensure_rec(c, tiles);
     if (rc) return rc;
     rc = ensure_r0(c, std::max<uint64_t>(nres, c->nsp));
     if (rc) return rc;
-    if (old_bufs[0] != c->d_rec || old_bufs[1] != c->d_r0 || old_bufs[2] != c->d_tinfo) drop_graphs(c);
+    if (mp.flags & MP_SPARSE) {
+        rc = ensure_wcopy(c, grid * FUSED_COPY_WORDS);
+        if (rc) return rc;
+    }
+    if (old_bufs[0] != c->d_rec || old_bufs[1] != c->d_r0 || old_bufs[2] != c->d_tinfo || old_bufs[3] != c->d_wcopy)
+        drop_graphs(c);
     HCK(c, hipMemcpyAsync(c->d_wins, c->h_wins, np * sizeof(WinDesc), hipMemcpyHostToDevice, s));
     HCK(c, hipMemcpyAsync(c->d_rec_lo, c->h_lohi, np * 8ull, hipMemcpyHostToDevice, s));
     HCK(c, hipMemcpyAsync(c->d_rec_hi, c->h_lohi + np, np * 8ull, hipMemcpyHostToDevice, s));
-    uint64_t grid = (uint64_t)c->cus * sh.wg;
-    if (grid > tiles) grid = tiles;
     SlowList sl = slow_list(c);
     const std::vector<uint64_t> key = {sh.wheel ? 1ull : 0ull, sh.tile_log2, sh.wg, sh.med_div, sh.wave_div,
                                        mp.flags, mp.proof_stride, np, tiles, grid, p_small, flags,
                                        (uint64_t)(uintptr_t)c->d_rec, (uint64_t)(uintptr_t)c->d_tinfo,
                                        (uint64_t)(uintptr_t)c->d_wins,
                                        (uint64_t)(uintptr_t)c->d_out, (uint64_t)(uintptr_t)c->d_r0,
-                                       (uint64_t)(uintptr_t)c->d_rec_lo, (uint64_t)(uintptr_t)c->d_rec_hi};
+                                       (uint64_t)(uintptr_t)c->d_rec_lo, (uint64_t)(uintptr_t)c->d_rec_hi,
+                                       (uint64_t)(uintptr_t)c->d_wcopy};
     hipGraphExec_t exec = nullptr;
     for (auto& g : c->graphs)
         if (g.key == key) exec = g.exec;
@@ -1121,7 +1137,7 @@ int run_launch(gb_ctx* c, const Shape& sh, const Piece* pcs, uint32_t np, uint64
             if (e == hipSuccess) e = launch_residues(s, T, c->d_wins, np, c->nsp, c->d_r0);
             if (e == hipSuccess)
                 e = launch_fused(s, T, mp, c->d_wins, np, tiles, te, (uint32_t)grid, c->d_counter, sl, c->d_rec,
-                                 c->d_r0);
+                                 c->d_r0, c->d_wcopy);
         }
         if (e == hipSuccess) e = launch_slow(s, T, mp, sl, c->d_slow_res, c->d_wins, nullptr, nullptr, 0);
         if (e == hipSuccess)

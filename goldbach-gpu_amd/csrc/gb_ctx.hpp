@@ -20,7 +20,7 @@ constexpr uint32_t PLAIN_TILE_BITS = 1u << 18;   // plain sieve tile (bits per w
 constexpr uint32_t LANE_ITEM_DIV = 32;            // default: lane items for p * 32 < tile bits (odd-only tiles)
 constexpr uint32_t WHEEL_LANE_ITEM_DIV = 128;     // ... and p * 128 < stream bits in the wheel
 constexpr uint32_t WAVE_ITEM_DIV = 1024;          // default: wave items for p * 1024 < stream bits
-constexpr uint32_t MR_PROOF_STRIDE = 3;           // default: odd-only MR tiles prove one window word in 3
+constexpr uint32_t MR_PROOF_STRIDE = 5;           // default: odd-only MR tiles prove one window word in 5
 
 struct ItemTable {
     uint32_t tb = 0;        // tile bits the schedule was cut for
@@ -83,6 +83,8 @@ struct gb_ctx {
     uint64_t* d_counter = nullptr;
     gbk::PrimeRes* d_r0 = nullptr;     // per-window sieving-prime entries
     uint64_t r0_cap = 0;
+    uint64_t* d_wcopy = nullptr;       // odd-only launches with sparse proofs: a saved window per workgroup
+    uint64_t wcopy_cap = 0;            // words
     // serving buffers
     uint64_t* d_qn = nullptr;
     uint64_t* d_qp = nullptr;

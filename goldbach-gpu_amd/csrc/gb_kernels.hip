@@ -125,14 +125,18 @@ __device__ uint32_t block_scan(uint32_t c, uint32_t tid, uint32_t* __restrict__ 
 // With stride > 1 only the survivors in every stride-th word are tested; the other words are
 // cleared, so a q left unproven reads as composite and the marking resolves an n only through a
 // proven prime (each n still meets about 65 / stride proven q among its 1,027 window primes).
+// copy (nullable): the pre-sieved window is saved there first, for resolve_open.
 template <int BS>
 __device__ void mr_tile(uint32_t* __restrict__ win32, uint32_t nbits, uint64_t qb, uint32_t tid,
-                        uint32_t* __restrict__ queue, uint32_t qcap, uint32_t* __restrict__ wsum, uint32_t stride) {
+                        uint32_t* __restrict__ queue, uint32_t qcap, uint32_t* __restrict__ wsum, uint32_t stride,
+                        uint64_t* __restrict__ copy) {
     const uint64_t* win64 = reinterpret_cast<const uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
     if (stride > 1) {
-        for (uint32_t k = tid; k < nw64; k += BS)
+        for (uint32_t k = tid; k < nw64; k += BS) {
+            if (copy) copy[k] = win64[k];
             if (k % stride) reinterpret_cast<uint64_t*>(win32)[k] = 0;
+        }
         __syncthreads();
     }
     for (uint32_t pass = 0; pass < 2; ++pass) {
@@ -189,11 +193,12 @@ __device__ __forceinline__ T ld32(const T* __restrict__ a, uint32_t i) {
 // MR = false leaves the Baillie-PSW step out of the instantiation, for windows whose q all lie
 // below (cover + 1)^2 (the callers check): the strong tests would otherwise set the register
 // budget, and so the occupancy, of every plain sieve launch.
+// Returns whether the tile's survivors went through the strong tests (q above the table's reach).
 template <int BS, bool MR = true>
-__device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ scratch, uint32_t scratch_cap,
+__device__ bool sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ scratch, uint32_t scratch_cap,
                            uint32_t* __restrict__ wsum, uint64_t qv0, uint32_t lead, uint64_t g0, uint32_t nbits,
                            const Tables& T, uint32_t tid, const PrimeRes* __restrict__ pr, uint64_t rbase,
-                           uint32_t proof_stride = 1) {
+                           uint32_t proof_stride = 1, uint64_t* __restrict__ copy = nullptr) {
     uint32_t* med_i0 = scratch;   // medium-prime first marks; the Miller-Rabin queue afterwards
     uint64_t* win64 = reinterpret_cast<uint64_t*>(win32);
     const uint32_t nw64 = (nbits + 63) >> 6;
@@ -333,9 +338,13 @@ __device__ void sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     // 4) above 2^44 the table cannot reach sqrt(q): survivors of the pre-sieve
     //    are decided by Baillie-PSW (exact below 2^64).
     if constexpr (MR) {
-        if (mr && any_valid)
-            mr_tile<BS>(win32, nbits, qv0 + 2 * g0 - 2ull * lead, tid, scratch, scratch_cap, wsum, proof_stride);
+        if (mr && any_valid) {
+            mr_tile<BS>(win32, nbits, qv0 + 2 * g0 - 2ull * lead, tid, scratch, scratch_cap, wsum, proof_stride,
+                        copy);
+            return true;
+        }
     }
+    return false;
 }
 
 // ---------------------------------------------------------------------------
@@ -531,12 +540,20 @@ __device__ __forceinline__ uint64_t word_valid(uint32_t w, uint32_t nw, uint32_t
     return (w == nw - 1 && (cnt & 63)) ? ((1ull << (cnt & 63)) - 1) : ~0ull;
 }
 
+// The n a tile with sparse proofs leaves open, kept in LDS for its second pass (resolve_open):
+// their offsets j (n = n_first + 2j); past cap they go to the slow list.
+struct OpenList {
+    uint32_t* j;
+    uint32_t* count;   // entries appended (may exceed cap)
+    uint32_t cap;
+};
+
 // Results of one word whose marking is complete: verified bits, Phase-2
-// count, and the n no window prime resolved (failures, or the slow list).
+// count, and the n no window prime resolved (failures, or the open or slow list).
 __device__ __forceinline__ void finish_word(uint32_t w, uint64_t v1, uint64_t v, uint64_t valid, uint64_t n_first,
                                             const MarkParams& mp, uint64_t* __restrict__ verified,
                                             uint32_t* __restrict__ pmin_out, const SlowList& sl, uint32_t win_idx,
-                                            LaneStats& st, uint32_t step = 2) {
+                                            LaneStats& st, uint32_t step = 2, const OpenList* ol = nullptr) {
     st.ph2 += __popcll(~v1 & valid);
     uint64_t u = ~v & valid;
     if (u) {
@@ -554,6 +571,13 @@ __device__ __forceinline__ void finish_word(uint32_t w, uint64_t v1, uint64_t v,
             while (u) {
                 const uint32_t b = __builtin_ctzll(u);
                 u &= u - 1;
+                if (ol) {
+                    const uint32_t o = atomicAdd(ol->count, 1u);
+                    if (o < ol->cap) {
+                        ol->j[o] = 64u * w + b;
+                        continue;
+                    }
+                }
                 const uint32_t slot = atomicAdd(sl.count, 1u);
                 if (slot < sl.cap) {
                     sl.n[slot] = n_first + (uint64_t)step * (64ull * w + b);
@@ -570,7 +594,7 @@ __device__ __forceinline__ void finish_word(uint32_t w, uint64_t v1, uint64_t v,
 template <int BS>
 __device__ void mark_words_fast(const uint64_t* __restrict__ win64, uint64_t n_first, uint32_t cnt,
                                 const MarkParams& mp, uint64_t* __restrict__ verified, const SlowList& sl,
-                                uint32_t win_idx, LaneStats& st, uint32_t tid) {
+                                uint32_t win_idx, LaneStats& st, uint32_t tid, const OpenList* ol) {
     const uint32_t lane = tid & 63, wave = tid >> 6;
     const uint32_t nw = (cnt + 63) >> 6;
     Pending P1, P2;
@@ -578,7 +602,7 @@ __device__ void mark_words_fast(const uint64_t* __restrict__ win64, uint64_t n_f
     P1.w0 = P1.w1 = P2.w0 = P2.w1 = 0;
     P1.v0 = P1.v1 = P2.v0 = P2.v1 = ~0ull;
     auto done = [&](uint32_t w, uint64_t v) {
-        finish_word(w, v, v, word_valid(w, nw, cnt), n_first, mp, verified, nullptr, sl, win_idx, st);
+        finish_word(w, v, v, word_valid(w, nw, cnt), n_first, mp, verified, nullptr, sl, win_idx, st, 2, ol);
     };
     // stage B on the 64 entries in P1's first slot; incomplete ones move to P2
     auto run_b = [&](uint32_t live) {
@@ -628,14 +652,15 @@ __device__ void mark_words_fast(const uint64_t* __restrict__ win64, uint64_t n_f
     }
 }
 
+// ol (nullable): the open list of a tile with sparse proofs (the unrolled path serves those).
 template <int BS>
 __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first, uint32_t cnt,
                            const MarkParams& mp, const uint32_t* __restrict__ sp, uint64_t* __restrict__ verified,
                            uint32_t* __restrict__ pmin_out, const SlowList& sl, uint32_t win_idx, LaneStats& st,
-                           uint32_t tid) {
+                           uint32_t tid, const OpenList* ol = nullptr) {
     const bool want = mp.flags & 1u;
     if (mp.K == NMARK && mp.p_small >= HALO_P && pmin_out == nullptr && !want) {
-        mark_words_fast<BS>(win64, n_first, cnt, mp, verified, sl, win_idx, st, tid);
+        mark_words_fast<BS>(win64, n_first, cnt, mp, verified, sl, win_idx, st, tid, ol);
         return;
     }
     const uint32_t lane = tid & 63, wave = tid >> 6;
@@ -684,6 +709,145 @@ __device__ void mark_words(const uint64_t* __restrict__ win64, uint64_t n_first,
         }
         if (!crossed) v1 = v;
         if (act) finish_word(w, v1, v, valid, n_first, mp, verified, pmin_out, sl, win_idx, st);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Second pass of a tile with sparse proofs.  An open n met no proven prime q = n - p with p
+// prime, so its other candidates are the pre-sieve survivors of the unproven words: q = n - p
+// with p <= 8191 prime and q's window bit set in a word left unproven.  A lane walks them in
+// p order by reading the saved window downward from q = n - 3 (one word per step, the set bits
+// from the top) and checking p against the bitset of odd primes; a group of L lanes
+// (L = BS / m, at most 8, inside one wave) takes an n and tests its candidates L at a time by
+// Baillie-PSW until one is prime (a survivor is prime about 70 % of the time), so a tile's few
+// open n keep its waves busy for a round or two.  win64 holds the saved pre-sieve window; an n
+// with no prime candidate goes to the slow list.
+// ---------------------------------------------------------------------------
+struct OpenWalk {
+    uint32_t kw;     // window word being read
+    uint32_t kend;   // the lowest word that may hold a candidate (p <= 8191)
+    uint64_t bits;   // its set bits not yet taken
+};
+
+// The walk of the open n = n_first + 2j from q = n - 3 (window bit top - 1, top = HALO + j) down to
+// q = n - 8191 (bit top - 4095).
+__device__ __forceinline__ OpenWalk open_walk(const uint64_t* __restrict__ win64, uint32_t top, uint32_t stride) {
+    OpenWalk wk;
+    wk.kw = (top - 1) >> 6;
+    wk.kend = (top - (HALO_P - 3) / 2) >> 6;
+    wk.bits = wk.kw % stride ? win64[wk.kw] & (~0ull >> (63 - ((top - 1) & 63))) : 0;
+    return wk;
+}
+
+// An open n no candidate resolved: to the slow list, which decides it exactly.
+__device__ __forceinline__ void open_to_slow(const SlowList& sl, uint64_t n, uint32_t win_idx) {
+    const uint32_t slot = atomicAdd(sl.count, 1u);
+    if (slot < sl.cap) {
+        sl.n[slot] = n;
+        sl.win[slot] = win_idx;
+    } else {
+        atomicOr(sl.overflow, 1u);
+    }
+}
+
+// Next candidate bit of the walk (p ascending), or ~0u when none is left.
+__device__ __forceinline__ uint32_t open_next(OpenWalk& wk, const uint64_t* __restrict__ win64, uint32_t top,
+                                              uint32_t stride, const uint32_t* __restrict__ podd) {
+    for (;;) {
+        while (wk.bits) {
+            const uint32_t t = 63 - __builtin_clzll(wk.bits);
+            wk.bits &= ~(1ull << t);
+            const uint32_t b = 64 * wk.kw + t;
+            const uint32_t h = top - b;   // (p - 1) / 2, p = 2h + 1
+            if (h > (HALO_P - 3) / 2) return ~0u;
+            if ((podd[h >> 5] >> (h & 31)) & 1) return b;
+        }
+        if (wk.kw == wk.kend) return ~0u;
+        --wk.kw;
+        wk.bits = wk.kw % stride ? win64[wk.kw] : 0;
+    }
+}
+
+template <int BS>
+__device__ void resolve_open(const uint64_t* __restrict__ win64, uint64_t n_first, const uint32_t* __restrict__ open_j,
+                             uint32_t m, uint32_t* __restrict__ next, uint32_t stride,
+                             const uint32_t* __restrict__ podd, const SlowList& sl, uint32_t win_idx, LaneStats& st,
+                             uint32_t tid) {
+    if (m >= BS) {
+        // more open n than lanes: a lane takes the next n from the list whenever its own is decided
+        bool have = false;
+        uint32_t top = 0;
+        uint64_t n = 0;
+        OpenWalk wk{};
+        auto take = [&]() {
+            const uint32_t i = atomicAdd(next, 1u);
+            have = i < m;
+            if (have) {
+                top = HALO + open_j[i];
+                n = n_first + 2ull * (top - HALO);
+                wk = open_walk(win64, top, stride);
+            }
+        };
+        take();
+        while (__any(have)) {
+            uint64_t q = 0;
+            bool has = false;
+            if (have) {
+                const uint32_t b = open_next(wk, win64, top, stride, podd);
+                if (b != ~0u) {
+                    q = n - (2ull * (top - b) + 1);
+                    has = true;
+                } else {
+                    open_to_slow(sl, n, win_idx);
+                    take();
+                }
+            }
+            if (has && mr_test(q, 0, false) && bpsw_lucas(q)) {
+                st.ph2 -= 1;   // counted unverified by the marking
+                take();
+            }
+        }
+        return;
+    }
+    const uint32_t L = m >= BS / 8 ? BS / m : 8u;   // lanes per open n
+    const uint32_t G = 64 / L;                      // groups per wave
+    const uint32_t lane = tid & 63, r = lane % L, lead = lane - r;
+    const uint64_t gmask = ((1ull << L) - 1) << lead;
+    for (uint32_t base = 0; base < m; base += (BS / 64) * G) {
+        const uint32_t i = base + (tid >> 6) * G + lane / L;
+        const bool mine = lane < G * L && i < m;
+        const uint32_t top = HALO + (mine ? open_j[i] : 0);
+        const uint64_t n = n_first + 2ull * (top - HALO);
+        OpenWalk wk = open_walk(win64, top, stride);
+        bool open = mine, found = false;
+        uint32_t skip = r;   // candidates this lane passes over before its next test
+        while (__any(open)) {
+            uint64_t q = 0;
+            bool has = false;
+            if (open) {
+                uint32_t b = open_next(wk, win64, top, stride, podd);
+                while (b != ~0u && skip) {
+                    --skip;
+                    b = open_next(wk, win64, top, stride, podd);
+                }
+                if (b != ~0u) {
+                    q = n - (2ull * (top - b) + 1);
+                    has = true;
+                    skip = L - 1;
+                } else {
+                    open = false;   // this lane's share of the candidates is used up
+                }
+            }
+            const bool prime = has && mr_test(q, 0, false) && bpsw_lucas(q);
+            if (__ballot(prime) & gmask) {
+                found = true;
+                open = false;
+            }
+        }
+        if (mine && r == 0) {
+            if (found) st.ph2 -= 1;   // counted unverified by the marking
+            else open_to_slow(sl, n, win_idx);
+        }
     }
 }
 
@@ -754,18 +918,35 @@ template <int BS>
 __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, const WinDesc* __restrict__ wins,
                                                    uint32_t nwin, uint64_t ntiles, uint32_t tile_evens,
                                                    uint64_t* __restrict__ tile_counter, SlowList sl,
-                                                   StatRec* __restrict__ rec, const PrimeRes* __restrict__ pr) {
+                                                   StatRec* __restrict__ rec, const PrimeRes* __restrict__ pr,
+                                                   uint64_t* __restrict__ wcopy) {
     // the window at a fixed LDS address (sized for the largest tile); dynamic
-    // LDS after it: [medium first marks | stats | tile id]
+    // LDS after it: [stats | tile id | scan words | open count | scratch]
     __shared__ __align__(16) uint32_t win32[FUSED_WIN_WORDS];
-    extern __shared__ uint32_t lds[];   // [stats | tile id | scan words | scratch]
+    extern __shared__ uint32_t lds[];
     LaneStats* red = reinterpret_cast<LaneStats*>(lds);
     uint64_t& s_tile = *reinterpret_cast<uint64_t*>(red + BS / 64);
     uint32_t* wsum = reinterpret_cast<uint32_t*>(&s_tile + 1);
-    uint32_t* scratch = wsum + BS / 64;   // medium first marks, then the Miller-Rabin queue
+    uint32_t* open_cnt = wsum + BS / 64;          // [open n appended | next open n to take]
+    uint32_t* podd = open_cnt + 2;                // bit h: 2h + 1 is an odd prime <= 8191
+    uint32_t* scratch = podd + (HALO_P + 63) / 64;   // medium first marks, the Miller-Rabin queue, then the open list
     const uint32_t tid = threadIdx.x;
+    // sparse proofs: this workgroup's slot for the saved pre-sieve window
+    uint64_t* const copy = (mp.flags & MP_SPARSE) ? wcopy + (uint64_t)blockIdx.x * FUSED_COPY_WORDS : nullptr;
+    if (copy) {
+        for (uint32_t k = tid; k < (HALO_P + 63) / 64; k += BS) podd[k] = 0;
+        __syncthreads();
+        for (uint32_t k = tid; k < NMARK; k += BS) {
+            const uint32_t h = T.sp[k] >> 1;
+            atomicOr(&podd[h >> 5], 1u << (h & 31));
+        }
+    }
     for (;;) {
-        if (tid == 0) s_tile = atomicAdd((unsigned long long*)tile_counter, 1ull);
+        if (tid == 0) {
+            s_tile = atomicAdd((unsigned long long*)tile_counter, 1ull);
+            open_cnt[0] = 0;
+            open_cnt[1] = 0;
+        }
         __syncthreads();
         const uint64_t t = s_tile;
         __syncthreads();
@@ -782,11 +963,23 @@ __global__ __launch_bounds__(BS) void fused_kernel(Tables T, MarkParams mp, cons
         const uint64_t rem = wd.evens - (uint64_t)tile_evens * lt;
         const uint32_t cnt = (uint32_t)(rem < tile_evens ? rem : tile_evens);
         const WinGeom g = window_geom(n0, (uint64_t)cnt + HALO + WIN_PAD);
-        sieve_tile<BS>(win32, scratch, FUSED_MRQ, wsum, g.qv0, g.lead, 0, g.nbits, T, tid, pr + wd.r0off, wd.rbase,
-                       (mp.flags & MP_SPARSE) ? mp.proof_stride : 1u);
+        const bool sparse = sieve_tile<BS>(win32, scratch, FUSED_MRQ, wsum, g.qv0, g.lead, 0, g.nbits, T, tid,
+                                           pr + wd.r0off, wd.rbase, copy ? mp.proof_stride : 1u, copy) && copy;
         LaneStats st = lane_stats_zero();
+        const OpenList ol{scratch, open_cnt, FUSED_MRQ};
         mark_words<BS>(reinterpret_cast<const uint64_t*>(win32), n0, cnt, mp, T.sp, nullptr, nullptr, sl, lo, st,
-                       tid);
+                       tid, sparse ? &ol : nullptr);
+        if (sparse) {
+            __syncthreads();   // the open list is complete
+            const uint32_t m = *open_cnt < FUSED_MRQ ? *open_cnt : FUSED_MRQ;
+            if (m) {
+                // the saved window back into LDS (written by this workgroup: the barriers since order it)
+                uint64_t* win64 = reinterpret_cast<uint64_t*>(win32);
+                for (uint32_t k = tid; k < (g.nbits + 63) >> 6; k += BS) win64[k] = copy[k];
+                __syncthreads();
+                resolve_open<BS>(win64, n0, scratch, m, open_cnt + 1, mp.proof_stride, podd, sl, lo, st, tid);
+            }
+        }
         block_store<BS>(st, cnt, &rec[t], tid, red);
     }
 }
@@ -1801,7 +1994,8 @@ hipError_t launch_residues(hipStream_t s, const Tables& T, const WinDesc* wins, 
 
 static size_t fused_dyn_lds(uint32_t n_med) {
     const uint32_t med = n_med > NPAT ? n_med - NPAT : 1;
-    return sizeof(LaneStats) * (FUSED_BS / 64) + 8 + (size_t)(FUSED_BS / 64 + (med > FUSED_MRQ ? med : FUSED_MRQ)) * 4;
+    return sizeof(LaneStats) * (FUSED_BS / 64) + 8 +
+           (size_t)(FUSED_BS / 64 + 2 + (HALO_P + 63) / 64 + (med > FUSED_MRQ ? med : FUSED_MRQ)) * 4;
 }
 
 size_t fused_lds_bytes(uint32_t tile_evens, uint32_t n_med) {
@@ -1811,13 +2005,14 @@ size_t fused_lds_bytes(uint32_t tile_evens, uint32_t n_med) {
 
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
-                        SlowList sl, StatRec* rec, const PrimeRes* pr) {
+                        SlowList sl, StatRec* rec, const PrimeRes* pr, uint64_t* wcopy) {
     if (tile_evens > FUSED_TILE_MAX || (tile_evens & 63)) return hipErrorInvalidValue;
+    if ((mp.flags & MP_SPARSE) && (!wcopy || mp.proof_stride < 2)) return hipErrorInvalidValue;
     const size_t lds = fused_dyn_lds(T.n_med);
     hipError_t e = allow_lds(fused_kernel<FUSED_BS>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fused_kernel<FUSED_BS>, dim3(grid), dim3(FUSED_BS), lds, s, T, mp, wins, nwin, ntiles,
-                       tile_evens, tile_counter, sl, rec, pr);
+                       tile_evens, tile_counter, sl, rec, pr, wcopy);
     return hipGetLastError();
 }
 

@@ -114,10 +114,13 @@ hipError_t launch_mark_segment(hipStream_t s, const Tables& T, const MarkParams&
 size_t fused_lds_bytes(uint32_t tile_evens, uint32_t n_med);
 constexpr size_t LDS_PER_CU = 160 * 1024;
 
-// Fused persistent kernel over the tiles of `nwin` windows.
+// Fused persistent kernel over the tiles of `nwin` windows.  With MP_SPARSE, wcopy holds
+// FUSED_COPY_WORDS words per workgroup of the grid: each tile saves its pre-sieved window there
+// before the strong tests, for the second pass over the n its proven q leave open.
 hipError_t launch_fused(hipStream_t s, const Tables& T, const MarkParams& mp, const WinDesc* wins, uint32_t nwin,
                         uint64_t ntiles, uint32_t tile_evens, uint32_t grid, uint64_t* tile_counter,
-                        SlowList sl, StatRec* rec, const PrimeRes* pr);
+                        SlowList sl, StatRec* rec, const PrimeRes* pr, uint64_t* wcopy);
+constexpr uint32_t FUSED_COPY_WORDS = FUSED_WIN_WORDS / 2;   // 64-bit words of one saved window
 
 // Wheel (mod 6) form of the fused fast path: tiles of 3 * cls_len even n (cls_len per
 // class n mod 6), the window kept as two streams of q = 6k + 1 and q = 6k + 5.

@@ -170,9 +170,10 @@ typedef struct gb_tuning {
                                are cut into wavefront items of 512 marks (0 = 1024; 512..65536) */
     uint32_t mr_proof_stride; /* batched, odd-only tiles above 2^44 without GB_WANT_PMIN: the strong
                                tests prove the survivors of one 64-bit window word in every s
-                               (0 = 3; 1 = every survivor; at most 64); an n that no
-                               proven q resolves is decided by the exact slow path, so the counts
-                               do not depend on s */
+                               (0 = 5; 1 = every survivor; at most 64); an n that no proven q
+                               resolves tries the other pre-sieve survivors q = n - p (p <= 8191)
+                               by Baillie-PSW, and what is still open is decided by the exact
+                               slow path, so the counts do not depend on s */
 } gb_tuning;
 
 #define GB_TUNE_AUTO 0x2u   /* gb_tuning.flags: take tile / segment size from gb_autotune (cached per context) */

@@ -168,12 +168,13 @@ def test_many_ragged_windows_span_several_launches(ctx, orc):
         assert (got["evens"], got["phase2"], got["failures"]) == (want["evens"], want["phase2"], want["failures"])
 
 
-@pytest.mark.parametrize("stride", [1, 2, 3, 64])
+@pytest.mark.parametrize("stride", [1, 2, 3, 5, 6, 8, 64])
 def test_mr_proof_stride_leaves_the_counts_alone(ctx, gb, orc, stride):
-    # odd-only tiles above 2^44 prove the survivors of one window word in `stride` (4 by default);
-    # an n that no proven q resolves goes to the slow path, which decides it exactly from p = 3
-    # (at stride 64 about a third of the n go there).  P_SMALL = 8193 makes the record n
-    # 3325581707333960528 (p_min 9781) a Phase-2 case.
+    # odd-only tiles above 2^44 prove the survivors of one window word in `stride`; an n that no
+    # proven q resolves gets a second pass over the pre-sieve survivors of the unproven words, and
+    # what that leaves goes to the slow path, which decides it exactly from p = 3 (at stride 64
+    # about a third of the n are open: more than a tile's open list holds).  P_SMALL = 8193
+    # makes the record n 3325581707333960528 (p_min 9781) a Phase-2 case.
     rec = 3325581707333960528
     for a, e, ps in ((10**15, 30_000, 10**6), (2**63 + 2, 20_000, 10**6), (2**64 - 2 - 2 * 19_999, 20_000, 10**6),
                      (rec - 2 * 5000, 10_000, 8193)):
