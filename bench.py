@@ -18,11 +18,16 @@ The JSON line carries:
                 HBM view: algorithmic bytes (0.375 B per even n: window bitset
                 written + read, verified bitmap written) over the same time, and
                 the measured traffic.  Without a current profile the HBM view is
-                the roofline.
+                the roofline.  roofline.lds prices the same pass's counted
+                SQ_INSTS_LDS (wave-level LDS instructions) against the LDS probe.
   step_spread   min / median / max / mean / stdev / CV of the timed passes
                 (the protocol of benchmarks/measure_variance.sh:11-28).
   valu_probe    issue rates of independent v_add_u32, independent v_mad_u64_u32
                 and a dependent v_add_u32 chain (lib/libgb_probe.so).
+  lds_probe     wave-level LDS instruction rates: ds_and_b32 at scattered and at
+                bank-disjoint words, ds_read_b32, ds_and_b32 from half the lanes; the
+                scattered ds_and_b32 rate is the ceiling of roofline.lds (the wheel
+                kernel's counted SQ_INSTS_LDS over its live time).
   plain         the plain form over [4, 1e12] (two segments in flight on two streams)
                 with its host overhead; beside it
                 [4, 1e11] in 2^27-even segments at P_SMALL = 1e6 and 4e9 (the cost
@@ -111,6 +116,8 @@ PROFILE_RUNS = {
     "hs": ("--pmc SQ_INSTS_VALU SQ_INSTS_VALU_INT32 SQ_INSTS_VALU_INT64 SQ_INSTS_SALU SQ_INSTS_LDS SQ_WAVES "
            "SQ_BUSY_CYCLES", WINDOWS_CMD),
     "p": ("--pmc SQ_INSTS_VALU SQ_BUSY_CYCLES SQ_WAVES", "python3 -u bench.py --valu-probe"),
+    "l": ("--pmc SQ_INSTS_LDS SQ_INSTS_VALU SQ_LDS_IDX_ACTIVE SQ_LDS_BANK_CONFLICT SQ_BUSY_CYCLES SQ_WAVES",
+          "python3 -u bench.py --lds-probe"),
 }
 COPY_PROBE_BYTES = 1 << 30
 
@@ -230,8 +237,13 @@ def summarize_profiles(out_dir: str) -> int:
         f.write("\n")
     probe = [(kname[kname.index("valu_probe"):kname.index(">") + 1], ctr, ns)
              for kname, ctr, ns in _dispatches(out_dir, "p") if "valu_probe" in kname]
+    lprobe = []
+    if os.path.isdir(os.path.join(out_dir, "l")):
+        lprobe = [(kname[kname.index("lds_probe"):kname.index(">") + 1], ctr, ns)
+                  for kname, ctr, ns in _dispatches(out_dir, "l") if "lds_probe" in kname]
     print(json.dumps({"wheel": prof, "windows": hprof, "windows_trace": htrace, "copy_kib": [cf_kib, cw_kib],
-                      "probe": [{"kernel": k, "ns": ns, **c} for k, c, ns in probe]}, indent=1))
+                      "probe": [{"kernel": k, "ns": ns, **c} for k, c, ns in probe],
+                      "lds_probe": [{"kernel": k, "ns": ns, **c} for k, c, ns in lprobe]}, indent=1))
     return 0
 
 
@@ -274,6 +286,33 @@ def valu_probe(kinds=(0, 1, 2), blocks: int = 256 * 64, iters: int = 2048) -> di
             rate = insts.value / (ms.value * 1e-3)
             best = rate if best is None else max(best, rate)
         out[names[k]] = {"wave_insts_per_s": best, "per_cu_clock_at_2.4GHz": best / (CUS * CLOCK_HZ)}
+    return out
+
+
+def lds_probe(kinds=(0, 1, 2, 3), iters: int = 2048) -> dict:
+    """Achievable LDS issue, measured: gb_probe_lds runs fixed counts of 32-bit LDS operations
+    from two 512-thread workgroups per CU (ds_and_b32 at scattered words, at words 64x + lane,
+    ds_read_b32 at scattered words, ds_and_b32 from half the lanes); rate = wave-level LDS
+    instructions / launch time (HIP events).  The scattered and the bank-disjoint addresses run
+    at nearly one rate, and half an exec mask saves little: the LDS cost is per wave-level
+    instruction, about six CU clocks each."""
+    lib = ctypes.CDLL(PROBE_LIB)
+    lib.gb_probe_lds.restype = ctypes.c_int
+    lib.gb_probe_lds.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    names = {0: "ds_and_b32_scattered", 1: "ds_and_b32_bank_disjoint", 2: "ds_read_b32_scattered",
+             3: "ds_and_b32_half_exec"}
+    out = {}
+    for k in kinds:
+        best = None
+        for _ in range(3):                                  # the first launch loads the code
+            ms, insts = ctypes.c_double(), ctypes.c_double()
+            rc = lib.gb_probe_lds(k, CUS * 2, iters, ctypes.byref(ms), ctypes.byref(insts))
+            if rc != 0:
+                raise RuntimeError(f"gb_probe_lds({k}) failed: {rc}")
+            rate = insts.value / (ms.value * 1e-3)
+            best = rate if best is None else max(best, rate)
+        out[names[k]] = {"wave_insts_per_s": best, "cu_clocks_per_inst_at_2.4GHz": CUS * CLOCK_HZ / best}
     return out
 
 
@@ -725,6 +764,8 @@ def main() -> int:
                     help="skip the reference's own cpu_goldbach 1000000000 (one thread, 10-60 s)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device (counter calibration for the traffic figure)")
+    ap.add_argument("--lds-probe", action="store_true",
+                    help="only run the LDS issue probes (for rocprofv3 SQ_INSTS_LDS / SQ_BUSY_CYCLES)")
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the VALU issue probes (for rocprofv3 SQ_INSTS_VALU / SQ_BUSY_CYCLES)")
     ap.add_argument("--profile-commands", metavar="DIR",
@@ -738,6 +779,9 @@ def main() -> int:
         return 0
     if args.summarize_profiles:
         return summarize_profiles(args.summarize_profiles)
+    if args.lds_probe:
+        print(json.dumps({"lds_probe": lds_probe(), "build_hash": build_hash()}))
+        return 0
     if args.valu_probe:
         print(json.dumps({"valu_probe": valu_probe(), "build_hash": build_hash()}))
         return 0
@@ -775,6 +819,8 @@ def main() -> int:
 
     probe = valu_probe()
     peak_valu = probe["v_add_u32_independent"]["wave_insts_per_s"]
+    lprobe = lds_probe()
+    peak_lds = lprobe["ds_and_b32_scattered"]["wave_insts_per_s"]
     prof = load_profile(TRAFFIC_FILE)         # counters of the current sources, or None
     traffic_pass = prof.get("hbm_bytes_per_pass") if prof else None
     traffic = traffic_pass / launches if traffic_pass else None           # HBM bytes per launch
@@ -786,6 +832,14 @@ def main() -> int:
         achieved_valu = prof["sq_insts_valu_per_pass"] / (kms * 1e-3)
         roofline = {"bound": "valu", "achieved": achieved_valu, "peak": peak_valu, "unit": "wave-instructions/s",
                     "frac": achieved_valu / peak_valu, "traffic": traffic, "hbm": hbm}
+        if prof.get("sq_insts_lds_per_pass"):
+            # the other half-used unit: wave-level LDS instructions (the sieve's marks and the
+            # marking's reads) against the rate the LDS probe reaches with scattered ds_and_b32
+            achieved_lds = prof["sq_insts_lds_per_pass"] / (kms * 1e-3)
+            roofline["lds"] = {"achieved": achieved_lds, "peak": peak_lds, "unit": "wave-instructions/s",
+                               "frac": achieved_lds / peak_lds,
+                               "counted": "SQ_INSTS_LDS of the pass's launches, " + os.path.relpath(TRAFFIC_FILE, ROOT),
+                               "peak_note": "scattered ds_and_b32 rate of gb_probe_lds, measured in this run"}
     else:
         roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
@@ -842,6 +896,7 @@ def main() -> int:
         "roofline": roofline,
         "step_spread": spread(per_step),
         "valu_probe": probe,
+        "lds_probe": lprobe,
         "cpu_baseline": cpu,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,

@@ -281,6 +281,8 @@ __device__ bool sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
         }
     }
 
+    // marks at raised issue priority, as in the wheel tiles
+    __builtin_amdgcn_s_setprio(1);
     // 3a) medium items: CHUNK marks each (unrolled when the item is whole);
     //     the next item's words are loaded before this item's marks issue
     const uint32_t n_items = kmed_end > NPAT ? T.item_start[kmed_end] : 0;
@@ -312,6 +314,7 @@ __device__ bool sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     // 3b) large primes, one per thread; the next prime's entry is loaded
     //     before this prime's marks issue.  Primes p <= sqrt(q_lo) need no
     //     p^2 adjustment (all but the last few of a tile).
+    __builtin_amdgcn_s_setprio(2);
     if (any_valid) {
         const uint64_t root_lo = isqrt64(q_lo);
         uint32_t k = T.n_med + tid;
@@ -333,6 +336,7 @@ __device__ bool sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
             k = kn;
         }
     }
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
 
     // 4) above 2^44 the table cannot reach sqrt(q): survivors of the pre-sieve
@@ -1084,6 +1088,10 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
         const int64_t i = (int64_t)(p / 6) - K0;
         if (i >= 0 && i < (int64_t)nbits) atomicOr(&wh[(p % 6 == 5 ? SW : 0) + (uint32_t)(i >> 5)], 1u << (i & 31));
     }
+    // The marks below are LDS-bound while the other workgroup on the CU is often in its
+    // VALU-bound class marking: raising these waves' issue priority keeps the LDS fed
+    // (measured: [4, 1e12] 159.7 -> 154.2 ms; the level itself hardly matters).
+    __builtin_amdgcn_s_setprio(1);
     // 3a) wave items: a wavefront takes 64 * WAVE_MARKS marks of one prime in
     //     one stream, lane l the marks l, l + 64, ...  Those sit 64p bits apart:
     //     the same bit of words 2p apart, so a lane's clear-mask never changes
@@ -1135,6 +1143,7 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
         }
     }
     // 3c) large primes, one per thread, both streams
+    __builtin_amdgcn_s_setprio(2);
     {
         const int64_t kroot = (int64_t)(isqrt64(6ull * (uint64_t)(K0 > 0 ? K0 : 0) + 1) );
         uint32_t k = T.n_med + tid;
@@ -1163,6 +1172,7 @@ __device__ void sieve_wheel_tile(uint32_t* __restrict__ wh, uint32_t* __restrict
             k = kn;
         }
     }
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
 }
 
