@@ -196,6 +196,13 @@ Tables make_tables(const gb_ctx* c, const ItemTable& it) {
     return T;
 }
 
+void release_plain2_stream(gb_ctx* c) {
+    if (!c->plain2.stream) return;
+    (void)hipStreamSynchronize(c->plain2.stream);
+    (void)hipStreamDestroy(c->plain2.stream);
+    c->plain2.stream = nullptr;
+}
+
 int normalise_range(gb_ctx* c, uint64_t* start, uint64_t* limit) {
     if (*limit & 1) --*limit;           // goldbach.cu:594
     if (*start < 4) *start = 4;         // goldbach.cu:604-605
@@ -584,8 +591,7 @@ bool ensure_plain2(gb_ctx* c, uint64_t cnt) {
     if (P.window && wwords <= P.window_cap && vwords <= P.verified_cap && nrec <= P.rec_cap) return true;
     free_plain2(c);
     const std::string err = c->err;
-    bool ok = (P.stream || hipStreamCreate(&P.stream) == hipSuccess) &&
-              dalloc(c, &P.window, wwords) == hipSuccess && dalloc(c, &P.verified, vwords) == hipSuccess &&
+    bool ok = dalloc(c, &P.window, wwords) == hipSuccess && dalloc(c, &P.verified, vwords) == hipSuccess &&
               dalloc(c, &P.rec, nrec) == hipSuccess && dalloc(c, &P.r0, c->nsp) == hipSuccess &&
               dalloc(c, &P.wins, 1) == hipSuccess && dalloc(c, &P.lohi, 2) == hipSuccess &&
               dalloc(c, &P.out, 1) == hipSuccess && dalloc(c, &P.slow_n, SLOW_CAP) == hipSuccess &&
@@ -1262,7 +1268,8 @@ int run_plain(gb_ctx* c, uint64_t start, uint64_t limit, uint64_t seg, const Mar
     // of one segment, or a context that times the kernels apart, keeps one stream)
     SegBufs B[2] = {ctx_bufs(c), ctx_bufs(c)};
     hipStream_t st[2] = {c->stream, c->stream};
-    if (first < (limit - start) / 2 + 1 && !(c->cfg.flags & GB_CFG_KERNEL_TIMES) && ensure_plain2(c, first)) {
+    if (first < (limit - start) / 2 + 1 && !(c->cfg.flags & GB_CFG_KERNEL_TIMES) && ensure_plain2(c, first) &&
+        (c->plain2.stream || hipStreamCreate(&c->plain2.stream) == hipSuccess)) {
         B[1] = plain2_bufs(c);
         st[1] = c->plain2.stream;
     }

@@ -194,4 +194,10 @@ gbk::Tables make_tables(const gb_ctx* c, const ItemTable& it);
 // with a bound other than the context's phase2_limit.
 int min_partition_bounded(gb_ctx* c, const uint64_t* n, uint32_t count, uint64_t bound, uint64_t* p_out);
 
+// Destroys the plain form's second stream if the context holds one (its buffers stay): a stream
+// kept idle holds one of the process's hardware queues (GPU_MAX_HW_QUEUES, 4 by default), so the
+// four lane streams of a server opened next would share queues and queue behind each other.
+// The next plain range that takes two segments in flight creates the stream again.
+void release_plain2_stream(gb_ctx* c);
+
 }  // namespace gbh

@@ -392,6 +392,7 @@ int gb_serve_open(gb_ctx* c, const gb_serve_config* cfg, gb_server** out) {
     if (e == hipSuccess) e = dalloc(c, &s->d_blocks, 4);
     // four lanes of (nearly) equal size, two per engine, laid out in the slot table in order
     uint32_t first = 0;
+    release_plain2_stream(c);   // an idle plain stream would share a hardware queue with a lane
     for (uint32_t q = 0; q < 4; ++q) {
         Engine& E = s->engines[q / 2];
         Lane& L = E.lanes[q % 2];

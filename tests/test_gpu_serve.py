@@ -76,6 +76,23 @@ def test_results_alone_and_in_the_mix(ctx, orc):
         assert orc.is_prime(int(p)) and orc.is_prime(int(n) - int(p))
 
 
+def test_server_after_and_before_plain_ranges(gb, orc):
+    # A plain range of several segments runs its second segment stream; opening a server on the
+    # same context releases that stream (so the server's four lane streams keep hardware queues
+    # of their own), and the next plain range creates it again.  Answers stay exact throughout.
+    with gb.Context(0, max_seg_evens=1 << 20) as c:
+        want_range = orc.verify_range(4, 10**7, 1 << 20, 10**6)
+        keys = ("evens", "phase2", "failures", "sum_pmin", "max_pmin", "argmax_n")
+        for _ in range(2):
+            got = c.verify_range(4, 10**7, 10**6, form=gb.GB_FORM_PLAIN, seg_evens=1 << 20, want_pmin=True)
+            assert all(got[k] == want_range[k] for k in keys), (got, want_range)
+            ns = serving_ns(500, seed=3)
+            with c.server() as srv:
+                assert np.array_equal(srv.run(ns), orc.min_partition_batch(ns))
+        got = c.verify_range(4, 10**7, 10**6, form=gb.GB_FORM_PLAIN, seg_evens=1 << 20, want_pmin=True)
+        assert all(got[k] == want_range[k] for k in keys)
+
+
 def test_long_query_keeps_its_slot_while_others_pass(ctx, orc):
     with ctx.server(chunks_per_step=1) as srv:
         long_id = srv.submit([LONG_N])
