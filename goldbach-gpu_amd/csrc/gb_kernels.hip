@@ -209,7 +209,7 @@ __device__ bool sieve_tile(uint32_t* __restrict__ win32, uint32_t* __restrict__ 
     const uint64_t q_top = any_valid ? q_lo + 2ull * (nbits - 1 - lb) : 3;
     const uint64_t s = isqrt64(q_top);
     const bool mr = s > T.cover;
-    const uint32_t np = count_le(T.sp, T.nsp, mr ? (uint64_t)MR_PRESIEVE : s);
+    const uint32_t np = count_le(T.sp, T.nsp, mr ? (uint64_t)(proof_stride > 1 ? MR_PRESIEVE_SPARSE : MR_PRESIEVE) : s);
     const double dbf = (double)((q_lo - rbase) >> 1);   // bits from rbase to q_lo, < 2^51
 
     // 1) primes 3..61 as per-word masks: bit j of word k is q(64k+j); the
