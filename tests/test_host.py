@@ -31,6 +31,13 @@ def test_abi_exports_every_declared_symbol(gb):
     assert sorted(gb.EXPORTS) == syms
 
 
+def test_probe_library_exports_the_issue_probes():
+    # bench.py's VALU and LDS ceilings come from this library (loaded only, no device call here)
+    lib = ctypes.CDLL(os.path.join(PKG_DIR, "lib", "libgb_probe.so"))
+    for s in ("gb_probe_valu", "gb_probe_lds"):
+        assert hasattr(lib, s), s
+
+
 def test_open_without_device_fails_loudly(gb):
     n = ctypes.c_int(0)
     if gb.library().gb_device_count(ctypes.byref(n)) == gb.GB_OK and n.value > 0:
